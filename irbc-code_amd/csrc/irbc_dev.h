@@ -1,0 +1,171 @@
+/*
+ * irbc_dev.h — device-side data layout shared by every kernel of the
+ * MI355X iRBC step.
+ *
+ * A context holds a batch of replicas packed replica-major in one set of
+ * device arrays.  DevRep is one replica's descriptor: its offsets into the
+ * packed arrays, its parameters, and the dynamic per-step state that device
+ * code advances (so a captured sequence of launches needs no host writes).
+ *
+ * Precision: solvent positions and velocities are FP32 in HBM and all
+ * arithmetic on them is FP64 in registers; per-cell momentum sums are exact
+ * 64-bit fixed point; the membrane, parasite and bonds are FP64 throughout.
+ */
+#ifndef IRBC_DEV_H
+#define IRBC_DEV_H
+
+#include <stdint.h>
+
+#define IRBC_WG 256            /* particle and cell tile */
+#define IRBC_MEM_WG 1024       /* one membrane workgroup per replica */
+#define IRBC_SCAN_BLOCK 4096   /* elements per scan block (256 threads x 16) */
+#define IRBC_MAX_CAND 16       /* ligand candidates per knob (3x3 window in practice) */
+#define IRBC_LLIST_PER_TRI 64  /* a triangle lies within 2a of at most 4x4x4 cells */
+#define IRBC_RANDNO 12345678   /* unbound-ligand sentinel (define.h:37) */
+#define IRBC_FIX_SCALE 1099511627776.0      /* 2^40: fixed-point unit of cell sums */
+#define IRBC_FIX_INV (1.0 / 1099511627776.0)
+
+/* status bits raised by device code, read by the host after a sync */
+#define IRBC_ST_NAN 1
+#define IRBC_ST_CAND_OVERFLOW 2
+#define IRBC_ST_LLIST_OVERFLOW 4
+#define IRBC_ST_RANGE 8
+
+/* counter-based stream ids (Philox4x32-10 counters: {index, stream, hi, lo}) */
+#define IRBC_RS_SHIFT 1u
+#define IRBC_RS_AXIS 2u
+#define IRBC_RS_WALL 3u
+#define IRBC_RS_MEM 4u
+#define IRBC_RS_BOND 5u
+
+struct DevRep {
+    /* ---- geometry and packing ---- */
+    int32_t Lx, Ly, Lz, ncell;
+    int64_t coff;      /* first cell in packed cell arrays */
+    int64_t coff1;     /* first entry of per-replica (ncell+1) offset arrays = coff + r */
+    int64_t poff;      /* first particle slot */
+    int32_t np;        /* particles */
+    int32_t tile0;     /* first particle tile of this replica */
+    int32_t ctile0;    /* first cell tile */
+    int32_t voff, nv, eoff, ne, toff, nt;
+    int32_t loff, nlig, nligx, nligy, lxoff, lyoff;
+    int64_t llent_off; /* first entry of this replica's cell->triangle entries */
+    int64_t llent_cap;
+    int32_t inc_off;   /* vertex incidence CSR: offsets at voff + r, entries at inc_ent_off */
+    int32_t inc_ent_off;
+    int64_t scr_off;   /* per-element force slots (double3) */
+    int32_t rep_index;
+    int32_t pad0;
+    /* ---- parameters ---- */
+    double navg, mass, Theta, dtmd, cosalpha, sinalpha, walls, ramp_speed;
+    int64_t ramp_steps;
+    int32_t csteps, uplist;
+    double kappasa, kappad, kappav, kappab, pmass, ks, dthr, FD, Fg, el0, elap;
+    double PRmass, paraR, onrate, offrate, Pi;
+    double cos_t0, sin_t0;  /* cos, sin of theta0*Pi/180 (host libm) */
+    double pon;             /* 1 - exp(-onrate*dtmd) (host libm) */
+    double Sarea0, Volume0, theta0;
+    /* ---- dynamic state advanced on the device ---- */
+    int64_t step;           /* index of the step about to run */
+    int32_t colsteps, upd;  /* counters after this step's increment */
+    int32_t collide, llist; /* this step runs a collision / a list update */
+    int32_t rotate_pending; /* collision done, solvent rotation fused into the next stream */
+    int32_t cur;            /* which particle buffer holds the state (0 = a, 1 = b) */
+    double wallspeed;
+    double shift[3];
+    double Sarea, Volume;
+    double para[3], paraV[3], paraft[3], parafdt[3];
+    int32_t paradx, parady, paraid;
+    int32_t tapes;          /* 1: streams come from the tape buffers */
+    uint64_t seed;
+    int32_t status;
+    int32_t bond_draws;     /* draws consumed by the last bond step */
+    int32_t nbonds;         /* bonded knobs after the last bond step */
+    int32_t pad1;
+    /* ---- particle buffers (packed arrays, this replica at poff) ---- */
+    float *xa, *ya, *za, *vxa, *vya, *vza;
+    float *xb, *yb, *zb, *vxb, *vyb, *vzb;
+    uint32_t *ida, *idb;    /* particle id | inside-flag << 31 */
+};
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+
+/* ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) ---- */
+struct u4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ u4 philox4x32_10(u4 c, uint32_t k0, uint32_t k1)
+{
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u;
+    const uint32_t W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        uint32_t hi0 = __umulhi(M0, c.x), lo0 = M0 * c.x;
+        uint32_t hi1 = __umulhi(M1, c.z), lo1 = M1 * c.z;
+        u4 n;
+        n.x = hi1 ^ c.y ^ k0;
+        n.y = lo1;
+        n.z = hi0 ^ c.w ^ k1;
+        n.w = lo0;
+        c = n;
+        k0 += W0;
+        k1 += W1;
+    }
+    return c;
+}
+
+/* two 53-bit uniforms in [0,1) from one Philox block */
+__device__ __forceinline__ void philox_u2(uint64_t seed, uint32_t stream, uint64_t index,
+                                          uint32_t sub, double &u0, double &u1)
+{
+    u4 c = {(uint32_t)index, stream, (uint32_t)(index >> 32), sub};
+    u4 r = philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint64_t a = ((uint64_t)r.x << 32 | r.y) >> 11;
+    uint64_t b = ((uint64_t)r.z << 32 | r.w) >> 11;
+    u0 = (double)a * (1.0 / 9007199254740992.0);
+    u1 = (double)b * (1.0 / 9007199254740992.0);
+}
+
+/* two standard normals (Box-Muller) from one Philox block */
+__device__ __forceinline__ void philox_n2(uint64_t seed, uint32_t stream, uint64_t index,
+                                          uint32_t sub, double &z0, double &z1)
+{
+    double u0, u1;
+    philox_u2(seed, stream, index, sub, u0, u1);
+    double r = sqrt(-2.0 * log(1.0 - u0)); /* 1-u0 in (0,1] */
+    double t = 6.283185307179586 * u1;
+    z0 = r * cos(t);
+    z1 = r * sin(t);
+}
+
+/* the reference's bbound() while-loop wrap (Src/solfuncs.cpp:603-622) */
+__device__ __forceinline__ double wrap_loop(double x, double l)
+{
+    if (x > l)
+        while (x > l) x -= l;
+    if (x < 0)
+        while (x < 0) x += l;
+    return x;
+}
+
+/* cindex[][][] lookup (maincode.cpp:112-136, define.h:330-334): the
+ * never-written slab at i = Lx, j = Ly or k = Lz+1 reads as cell 0 */
+__device__ __forceinline__ int32_t cell_of(int fx, int fy, int fz, int Lx, int Ly, int Lz)
+{
+    if (fx >= 0 && fx < Lx && fy >= 0 && fy < Ly && fz >= 0 && fz <= Lz)
+        return (fx * Ly + fy) * (Lz + 1) + fz;
+    return 0;
+}
+
+__device__ __forceinline__ void cross3(const double a[3], const double b[3], double c[3])
+{
+    c[0] = (a[1] * b[2]) - (b[1] * a[2]);
+    c[1] = (a[2] * b[0]) - (b[2] * a[0]);
+    c[2] = (a[0] * b[1]) - (b[0] * a[1]);
+}
+
+#endif /* __HIPCC__ */
+
+#endif /* IRBC_DEV_H */
