@@ -1,0 +1,167 @@
+/*
+ * irbc.h — C ABI of the MI355X-native iRBC adhesive-dynamics step.
+ *
+ * One context owns a batch of independent replicas on one GPU; a single
+ * replica (the reference's whole program) is a batch of one.  All state
+ * lives in device memory; the host side of this ABI moves plain host arrays
+ * in and out (SoA, the reference's units, FP64 on the host side) and
+ * launches hand-written HIP kernels for gfx950.
+ *
+ * Every call returns an int status: IRBC_OK (0) or a negative IRBC_ERR_*.
+ * No exceptions, no exit().  A context is used by one host thread; calls are
+ * ordered on the context's HIP stream.
+ *
+ * Reference surface replaced (citations are to the reference sources):
+ *   define.h:35-73 macros + maincode.cpp literals   -> irbc_params (irbc_params.h)
+ *   define.h:182-429 globals (solvent, Cells, vts,
+ *     edges, tries, ligands, parasite)               -> irbc_ctx + set/get calls
+ *   CollisionStep()        Include/solfuncs.h:21     -> irbc_collision_step
+ *   llistupdate()          Include/solfuncs.h:22     -> irbc_llist_update
+ *   reflectionFed()/Solve()/bbound(), the streaming
+ *     loop maincode.cpp:510-562                      -> irbc_stream
+ *   TotalVA()              Include/memfuncs.h:25     -> irbc_membrane_va
+ *   force_calculate_dt(), pfdt(),
+ *     maincode.cpp:606-665                           -> irbc_membrane_forces
+ *   bond_dynamics() maincode.cpp:16 + :668-723       -> irbc_bond_step
+ *   the time loop maincode.cpp:461-764               -> irbc_md_steps
+ *   sortsolvent_initial1() Include/solfuncs.h:20     -> irbc_classify_solvent
+ *   TotalVA0/makeiedge/centerofmass + maincode.cpp
+ *     :228-375 mesh setup                            -> irbc_set_mesh
+ *   CRandomMersenne / StochasticLib1 streams         -> irbc_set_*_tape (parity
+ *                                                       harness) or the built-in
+ *                                                       counter-based streams
+ */
+#ifndef IRBC_H
+#define IRBC_H
+
+#include <stdint.h>
+
+#include "irbc_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IRBC_OK 0
+#define IRBC_ERR_ARG -1      /* bad argument (null, out of range, size mismatch) */
+#define IRBC_ERR_OOM -2      /* device or host allocation failed */
+#define IRBC_ERR_HIP -3      /* a HIP runtime call failed */
+#define IRBC_ERR_STATE -4    /* call out of order (e.g. no mesh loaded) */
+#define IRBC_ERR_DEVICE -5   /* device code raised a status flag (NaN, overflow) */
+#define IRBC_ERR_BUDGET -6   /* request does not fit the memory budget */
+#define IRBC_ERR_NODEVICE -7 /* no HIP device */
+
+typedef struct irbc_ctx irbc_ctx;
+
+/* One replica of a batch: its parameters and its sizes. */
+typedef struct irbc_replica_spec {
+    irbc_params params;
+    int64_t n_solvent;
+    int32_t n_vertices, n_edges, n_triangles;
+    int32_t pad;
+    uint64_t seed; /* key of the counter-based streams */
+} irbc_replica_spec;
+
+/* ---- context ---- */
+int irbc_create(const irbc_replica_spec *specs, int n_replicas, int device, irbc_ctx **out);
+int irbc_destroy(irbc_ctx *ctx);
+const char *irbc_last_error(void);
+int irbc_device_bytes(const irbc_ctx *ctx, int64_t *bytes);
+/* bytes a batch of these replicas would take on the device (admission) */
+int64_t irbc_estimate_bytes(const irbc_replica_spec *specs, int n_replicas);
+int irbc_sync(irbc_ctx *ctx);  /* waits, then reports device status flags */
+
+/* ---- scenario setup ---- */
+/* mesh in box coordinates; edges (ne x 2) and triangles (nt x 3), 0-based.
+ * Builds hinge partners, rest lengths, kappap, rest areas, A0, V0, theta0
+ * and hinge orientation as maincode.cpp:228-387 does. */
+int irbc_set_mesh(irbc_ctx *ctx, int r, const double *xyz, const int32_t *edges,
+                  const int32_t *tris);
+/* any pointer may be NULL to leave that field as it is */
+int irbc_set_vertices(irbc_ctx *ctx, int r, const double *x, const double *y, const double *z,
+                      const double *vx, const double *vy, const double *vz, const double *fx,
+                      const double *fy, const double *fz, const int32_t *ix, const int32_t *iy,
+                      const int32_t *pid, const int32_t *bondi, const int32_t *bondl,
+                      const int32_t *knob);
+int irbc_get_vertices(irbc_ctx *ctx, int r, double *x, double *y, double *z, double *vx,
+                      double *vy, double *vz, double *fx, double *fy, double *fz, double *poff,
+                      double *pon, int32_t *ix, int32_t *iy, int32_t *pid, int32_t *bondi,
+                      int32_t *bondl, int32_t *knob);
+/* solvent state in particle order; indio may be NULL (all outside) */
+int irbc_set_solvent(irbc_ctx *ctx, int r, int64_t n, const double *x, const double *y,
+                     const double *z, const double *vx, const double *vy, const double *vz,
+                     const int32_t *indio);
+/* positions/velocities as stored (FP32, widened), last shifted-cell key and
+ * inside flag, all in particle order; any pointer may be NULL */
+int irbc_get_solvent(irbc_ctx *ctx, int r, double *x, double *y, double *z, double *vx,
+                     double *vy, double *vz, int32_t *key, int32_t *indio);
+/* parasite: d[12] = pos, vel, force(t), force(t+dt); i[3] = image x, image y, cell */
+int irbc_set_parasite(irbc_ctx *ctx, int r, const double *d, const int32_t *i);
+int irbc_get_parasite(irbc_ctx *ctx, int r, double *d, int32_t *i);
+int64_t irbc_n_ligands(const irbc_ctx *ctx, int r);
+int irbc_set_ligands(irbc_ctx *ctx, int r, const int32_t *ligi);
+int irbc_get_ligands(irbc_ctx *ctx, int r, int32_t *ligi);
+/* g[10] = Sarea, Volume, Sarea0, Volume0, theta0, wallspeed, status,
+ * bond draws of the last step, step, bonded knobs */
+int irbc_get_globals(irbc_ctx *ctx, int r, double *g);
+/* the time-loop counters before the top-of-step increment (maincode.cpp:450-475) */
+int irbc_set_counters(irbc_ctx *ctx, int r, int64_t step, int32_t colsteps, int32_t upd);
+int irbc_get_counters(irbc_ctx *ctx, int r, int64_t *step, int32_t *colsteps, int32_t *upd);
+int irbc_set_wallspeed(irbc_ctx *ctx, int r, double wallspeed);
+
+/* ---- random streams ---- */
+int irbc_set_seed(irbc_ctx *ctx, int r, uint64_t seed);
+/* parity harness: the next collision / bond step reads these tapes.  Layout:
+ * shift_u[3]; cell_u[2*ncell]; wall_z[3*ncell]; mem_z[3*ncell]; bond u[n]
+ * (see oracle/oracle.h for the draw order they stand for) */
+int irbc_use_tapes(irbc_ctx *ctx, int r, int on);
+int irbc_set_collision_tape(irbc_ctx *ctx, int r, const double *shift_u, const double *cell_u,
+                            const double *wall_z, const double *mem_z);
+int irbc_set_bond_tape(irbc_ctx *ctx, int r, const double *u, int64_t n);
+
+/* ---- stages (every replica of the batch) ---- */
+int irbc_collision_step(irbc_ctx *ctx);
+int irbc_llist_update(irbc_ctx *ctx);
+int irbc_stream(irbc_ctx *ctx);
+int irbc_classify_solvent(irbc_ctx *ctx);
+#define IRBC_PH_KICK1 1
+#define IRBC_PH_DRIFT 2
+#define IRBC_PH_VA 4
+#define IRBC_PH_FORCES 8
+#define IRBC_PH_EXTERNAL 16
+#define IRBC_PH_BONDS 32
+#define IRBC_PH_KICK2 64
+#define IRBC_PH_COUNTERS 128
+int irbc_membrane_phases(irbc_ctx *ctx, int phases);
+int irbc_membrane_va(irbc_ctx *ctx, int r, double *area, double *volume);
+int irbc_membrane_forces(irbc_ctx *ctx);
+int irbc_bond_step(irbc_ctx *ctx);
+/* per-cell results of the last collision: vcm[3*ncell], nc, ncc, rand1,
+ * rand2, dvv, ThE; any pointer may be NULL */
+int irbc_get_cells(irbc_ctx *ctx, int r, double *vcm, int32_t *nc, double *ncc, double *rand1,
+                   double *rand2, double *dvv, double *the);
+/* cell -> triangle lists: off[ncell+1], ent[cap]; returns entries or < 0 */
+int64_t irbc_get_llist(irbc_ctx *ctx, int r, int64_t *off, int32_t *ent, int64_t cap);
+/* ligand candidates of the last bond step: cnt[nv], lig[cap]; returns total */
+int64_t irbc_get_pairs(irbc_ctx *ctx, int r, int32_t *cnt, int32_t *lig, int64_t cap);
+
+/* ---- the time loop ---- */
+int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
+/* keep per-cell results (irbc_get_cells) also for collisions run inside
+ * irbc_md_steps; costs ~56 B per cell per collision */
+int irbc_set_diagnostics(irbc_ctx *ctx, int on);
+
+/* ---- measurement: HIP-event timing on the context's stream ---- */
+/* every > 0: bracket the launches of every every-th step of irbc_md_steps
+ * with events; 0: off.  Resets the accumulators. */
+int irbc_profile(irbc_ctx *ctx, int every);
+/* t[0] = total ms of timed stream launches, t[1] = their count,
+ * t[2] = total ms of collision sequences, t[3] = their count,
+ * t[4] = total ms of membrane launches, t[5] = their count */
+int irbc_profile_read(irbc_ctx *ctx, double *t);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* IRBC_H */

@@ -1,0 +1,69 @@
+"""Build the MI355X library, the host driver and the CPU oracle, in-tree.
+
+    python irbc-code_amd/build.py
+
+Outputs:
+  irbc-code_amd/libirbc_amd.so   HIP kernels + C ABI (hipcc, gfx950)
+  irbc-code_amd/irbc_main        reference-style command-line driver
+  irbc-code_amd/librefnames.so   reference-named functions over the C ABI
+  oracle/liboracle.so            CPU checker (test infrastructure)
+  oracle/_ref/ref_run            harnessed reference build (only where the
+                                 reference sources are present)
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(HERE)
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+# -ffp-contract=off: no FMA contraction, so FP64 arithmetic rounds like the
+# reference's expression by expression; packed FP32 ops disabled on purpose.
+HIPFLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-ffp-contract=off",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wno-unused-result", "-Wno-unused-value"]
+
+
+def run(cmd, cwd=None):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.check_call(cmd, cwd=cwd)
+
+
+def build_lib():
+    out = os.path.join(HERE, "libirbc_amd.so")
+    tmp = out + ".tmp"
+    run([HIPCC] + HIPFLAGS + ["-shared", "-o", tmp, os.path.join(HERE, "csrc", "irbc.hip")])
+    os.replace(tmp, out)
+    return out
+
+
+def build_host():
+    inc = os.path.join(REPO, "include")
+    lib = ["-L" + HERE, "-lirbc_amd", "-Wl,-rpath,$ORIGIN"]
+    src_main = os.path.join(HERE, "host", "irbc_main.cpp")
+    src_host = os.path.join(HERE, "host", "scenario.cpp")
+    if os.path.exists(src_main):
+        run(["g++", "-O2", "-std=c++17", "-I" + inc, "-o", os.path.join(HERE, "irbc_main"),
+             src_main, src_host] + lib)
+    src_ref = os.path.join(HERE, "host", "refnames.cpp")
+    if os.path.exists(src_ref):
+        run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + inc, "-o",
+             os.path.join(HERE, "librefnames.so"), src_ref, src_host] + lib)
+
+
+def build_oracle():
+    run(["make", "-C", os.path.join(REPO, "oracle"), "-j8"])
+
+
+def build_all():
+    build_lib()
+    build_host()
+    build_oracle()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1:] or ["all"]
+    for w in what:
+        {"all": build_all, "lib": build_lib, "host": build_host, "oracle": build_oracle}[w]()

@@ -1,0 +1,1172 @@
+/*
+ * irbc.hip — host side of the C ABI (include/irbc.h) and the kernels it
+ * launches, built into libirbc_amd.so for gfx950.
+ *
+ * Layout in HBM (packed replica-major; each replica's particle slots start
+ * on a tile boundary so every workgroup belongs to exactly one replica):
+ *   solvent   x y z vx vy vz FP32 SoA, double-buffered for the counting sort,
+ *             + id|inside-flag u32, shifted key and rank i32      ~40 B/slot
+ *   cells     count/start i32, SRD table (vcm, axis, scale) 64 B, vertex list
+ *             head, cell->triangle CSR                            ~80 B/cell
+ *   membrane  FP64 vertex state, topology, per-element force slots
+ *   ligands   occupancy i32 and grid coordinate tables
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <iterator>
+#include <string>
+#include <vector>
+
+#include "../../include/irbc.h"
+#include "irbc_dev.h"
+
+#include "k_solvent.inc"
+#include "k_membrane.inc"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(call)                                                                    \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(IRBC_ERR_HIP, "%s: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                            \
+    } while (0)
+
+#define LAUNCHCHK()                                                                       \
+    do {                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                \
+        if (e_ != hipSuccess)                                                             \
+            return fail(IRBC_ERR_HIP, "kernel launch: %s (%s:%d)", hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                              \
+    } while (0)
+
+struct HostRep {
+    irbc_replica_spec spec;
+    int64_t step = 0;
+    int32_t colsteps = -1, upd = 0; /* reference counters before the top-of-step increment */
+    bool mesh = false, solvent = false;
+    std::vector<int32_t> e1, e2, t1, t2, t3;
+};
+
+struct Sizes {
+    int64_t slots = 0, cells = 0, cells1 = 0, llent = 0, scr = 0;
+    int32_t nv = 0, ne = 0, nt = 0, nlig = 0, nligx = 0, nligy = 0, inc = 0;
+    int32_t tiles = 0, ctiles = 0, scanblk = 0;
+};
+
+int32_t lig_count(double L, double sp)
+{
+    int32_t n = 0;
+    double m = 0.0;
+    while (m < L) {
+        ++n;
+        m += sp;
+    }
+    return n;
+}
+
+/* upper bound of vertex incidences: 2 per edge end, 3 per triangle corner, 1 per hinge corner */
+int32_t inc_bound(const irbc_replica_spec &s) { return 4 * s.n_edges + 9 * s.n_triangles + 4 * s.n_edges; }
+
+Sizes plan(const irbc_replica_spec *specs, int n, std::vector<DevRep> *reps,
+           std::vector<ScanBlk> *blks, std::vector<int32_t> *rep_first_blk,
+           std::vector<int32_t> *tile_rep, std::vector<int32_t> *ctile_rep)
+{
+    Sizes z;
+    for (int r = 0; r < n; ++r) {
+        const irbc_replica_spec &s = specs[r];
+        const irbc_params &p = s.params;
+        DevRep R;
+        std::memset(&R, 0, sizeof(R));
+        R.Lx = p.Lx;
+        R.Ly = p.Ly;
+        R.Lz = p.Lz;
+        R.ncell = (int32_t)irbc_ncells(&p);
+        R.coff = z.cells;
+        R.coff1 = z.cells1;
+        R.tile0 = z.tiles;
+        R.poff = (int64_t)z.tiles * IRBC_WG;
+        R.np = (int32_t)s.n_solvent;
+        R.ctile0 = z.ctiles;
+        R.voff = z.nv;
+        R.nv = s.n_vertices;
+        R.eoff = z.ne;
+        R.ne = s.n_edges;
+        R.toff = z.nt;
+        R.nt = s.n_triangles;
+        R.nligx = lig_count(p.Lx, p.lig_spacing);
+        R.nligy = lig_count(p.Ly, p.lig_spacing);
+        R.nlig = R.nligx * R.nligy;
+        R.loff = z.nlig;
+        R.lxoff = z.nligx;
+        R.lyoff = z.nligy;
+        R.llent_off = z.llent;
+        R.llent_cap = (int64_t)IRBC_LLIST_PER_TRI * s.n_triangles;
+        R.inc_off = z.nv + r;
+        R.inc_ent_off = z.inc;
+        R.scr_off = z.scr;
+        R.rep_index = r;
+        const int32_t tiles = (int32_t)((s.n_solvent + IRBC_WG - 1) / IRBC_WG);
+        const int32_t ctiles = (R.ncell + IRBC_WG - 1) / IRBC_WG;
+        if (tile_rep) tile_rep->insert(tile_rep->end(), tiles, r);
+        if (ctile_rep) ctile_rep->insert(ctile_rep->end(), ctiles, r);
+        const int32_t nb = (R.ncell + IRBC_SCAN_BLOCK - 1) / IRBC_SCAN_BLOCK;
+        if (rep_first_blk) rep_first_blk->push_back(z.scanblk);
+        for (int b = 0; b < nb && blks; ++b) {
+            ScanBlk B;
+            B.rep = r;
+            B.first = b * IRBC_SCAN_BLOCK;
+            B.count = std::min(IRBC_SCAN_BLOCK, R.ncell - B.first);
+            B.bidx = b;
+            B.bfirst = z.scanblk;
+            B.nblk = nb;
+            blks->push_back(B);
+        }
+        z.scanblk += nb;
+        z.tiles += tiles;
+        z.ctiles += ctiles;
+        z.slots = (int64_t)z.tiles * IRBC_WG;
+        z.cells += R.ncell;
+        z.cells1 += R.ncell + 1;
+        z.nv += s.n_vertices;
+        z.ne += s.n_edges;
+        z.nt += s.n_triangles;
+        z.nlig += R.nlig;
+        z.nligx += R.nligx;
+        z.nligy += R.nligy;
+        z.llent += R.llent_cap;
+        z.scr += 8 * (int64_t)s.n_edges + 9 * (int64_t)s.n_triangles;
+        z.inc += inc_bound(s);
+        if (reps) reps->push_back(R);
+    }
+    if (rep_first_blk) rep_first_blk->push_back(z.scanblk);
+    return z;
+}
+
+int64_t bytes_of(const Sizes &z, int n)
+{
+    int64_t b = 0;
+    b += z.slots * (2 * 6 * 4 + 2 * 4 + 3 * 4);           /* particles x2, ids x2, keys, ranks, sorted keys */
+    b += z.cells * (4 + 8 * 8 + 4 + 4 + 8 * 8 + 4);       /* count, ctab, vhead, llcount, diag, cursor */
+    b += z.cells * 8 * 8;                                 /* tapes */
+    b += z.cells1 * (4 + 4);                              /* start, lloff */
+    b += z.llent * 4;
+    b += (int64_t)z.nv * (15 * 8 + 12 * 4 + IRBC_MAX_CAND * (4 + 8));
+    b += (int64_t)z.ne * (5 * 4 + 2 * 8) + (int64_t)z.nt * (3 * 4 + 2 * 8);
+    b += z.scr * 24 + (int64_t)z.inc * 4 + (int64_t)(z.nv + n) * 4;
+    b += (int64_t)z.nlig * 4 + (int64_t)(z.nligx + z.nligy) * 8;
+    b += (int64_t)z.scanblk * 4 + (int64_t)n * sizeof(DevRep);
+    return b;
+}
+
+} // namespace
+
+struct irbc_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    int nrep = 0;
+    std::vector<HostRep> hr;
+    std::vector<DevRep> rep;
+    Sizes z;
+    int64_t bytes = 0;
+    std::vector<void *> allocs;
+
+    DevRep *d_rep = nullptr;
+    int32_t *d_tile_rep = nullptr, *d_ctile_rep = nullptr, *d_rep_first_blk = nullptr;
+    ScanBlk *d_blk = nullptr;
+    int32_t *bsum = nullptr;
+    /* solvent */
+    float *xa, *ya, *za, *vxa, *vya, *vza, *xb, *yb, *zb, *vxb, *vyb, *vzb;
+    uint32_t *ida, *idb;
+    int32_t *skey, *srank, *skey2;
+    /* cells */
+    int32_t *ccount, *cstart, *vhead, *lloff, *llcount, *llent;
+    double *ctab;
+    CellDiag diag;
+    double *cell_u, *wall_z, *mem_z;
+    /* membrane */
+    double *mx, *my, *mz, *mvx, *mvy, *mvz, *mfx, *mfy, *mfz, *poff, *pon;
+    int32_t *mix, *miy, *mpid, *bondi, *bondl, *knob, *vnext;
+    int32_t *e1, *e2, *es1, *es2, *iedge, *t1, *t2, *t3;
+    double *l0, *kappap, *tarea, *tri_area, *wca, *slot;
+    int32_t *inc_off, *inc_ent;
+    int32_t *ncand, *cand, *draw_off, *flag, *list;
+    int32_t *ligi;
+    double *ligx, *ligy, *bond_u;
+    /* staging for downloads */
+    float *stage_f;
+    int32_t *stage_i;
+    bool diag_on = false;
+    /* profiling: HIP events around every prof_every-th step's launches */
+    bool prof = false;
+    int prof_every = 0;
+    int64_t prof_step = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pend[3];
+    double acc_ms[3] = {0, 0, 0};
+    int64_t acc_n[3] = {0, 0, 0};
+
+    template <class T> int alloc(T *&p, int64_t n)
+    {
+        p = nullptr;
+        if (n <= 0) n = 1;
+        void *q = nullptr;
+        if (hipMalloc(&q, (size_t)n * sizeof(T)) != hipSuccess)
+            return fail(IRBC_ERR_OOM, "hipMalloc of %lld bytes failed", (long long)(n * sizeof(T)));
+        if (hipMemsetAsync(q, 0, (size_t)n * sizeof(T), st) != hipSuccess)
+            return fail(IRBC_ERR_HIP, "hipMemsetAsync failed");
+        allocs.push_back(q);
+        bytes += n * (int64_t)sizeof(T);
+        p = (T *)q;
+        return IRBC_OK;
+    }
+
+    MemDev mem() const
+    {
+        MemDev M;
+        M.x = mx; M.y = my; M.z = mz; M.vx = mvx; M.vy = mvy; M.vz = mvz;
+        M.fx = mfx; M.fy = mfy; M.fz = mfz; M.poff = poff; M.pon = pon;
+        M.ix = mix; M.iy = miy; M.pid = mpid; M.bondi = bondi; M.bondl = bondl; M.knob = knob;
+        M.e1 = e1; M.e2 = e2; M.es1 = es1; M.es2 = es2; M.iedge = iedge;
+        M.t1 = t1; M.t2 = t2; M.t3 = t3;
+        M.l0 = l0; M.kappap = kappap; M.tarea = tarea;
+        M.tri_area = tri_area; M.wca = wca; M.slot = slot;
+        M.inc_off = inc_off; M.inc_ent = inc_ent;
+        M.ncand = ncand; M.cand = cand; M.draw_off = draw_off; M.flag = flag; M.list = list;
+        M.ligi = ligi; M.ligx = ligx; M.ligy = ligy; M.bond_u = bond_u;
+        return M;
+    }
+    MemView view() const
+    {
+        MemView V;
+        V.x = mx; V.y = my; V.z = mz; V.vx = mvx; V.vy = mvy; V.vz = mvz;
+        V.fx = mfx; V.fy = mfy; V.fz = mfz; V.t1 = t1; V.t2 = t2; V.t3 = t3;
+        return V;
+    }
+    int pull(int r)
+    {
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipMemcpy(&rep[r], d_rep + r, sizeof(DevRep), hipMemcpyDeviceToHost));
+        return IRBC_OK;
+    }
+    int push(int r)
+    {
+        HIPCHK(hipMemcpyAsync(d_rep + r, &rep[r], sizeof(DevRep), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        return IRBC_OK;
+    }
+    hipEvent_t event()
+    {
+        if (!ev_pool.empty()) {
+            hipEvent_t e = ev_pool.back();
+            ev_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        hipEventCreate(&e);
+        return e;
+    }
+    int flush_prof()
+    {
+        HIPCHK(hipStreamSynchronize(st));
+        for (int k = 0; k < 3; ++k) {
+            for (auto &pr : pend[k]) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+                acc_ms[k] += ms;
+                acc_n[k] += 1;
+                ev_pool.push_back(pr.first);
+                ev_pool.push_back(pr.second);
+            }
+            pend[k].clear();
+        }
+        return IRBC_OK;
+    }
+};
+
+/* top-of-step increment of the time loop (maincode.cpp:463-484, 501-505) */
+static void top_of_step(const irbc_params &p, int64_t step, int32_t colsteps_pre, int32_t upd_pre,
+                        DevRep &R)
+{
+    R.step = step;
+    R.wallspeed = irbc_wallspeed(&p, step);
+    R.colsteps = colsteps_pre + 1;
+    R.upd = upd_pre + 1;
+    R.collide = R.colsteps == p.csteps;
+    if (R.collide) R.colsteps = 0;
+    R.llist = R.upd == p.uplist;
+    if (R.llist) R.upd = 0;
+}
+
+static void advance_host(const irbc_params &p, HostRep &h)
+{
+    int32_t c = h.colsteps + 1, u = h.upd + 1;
+    if (c == p.csteps) c = 0;
+    if (u == p.uplist) u = 0;
+    h.colsteps = c;
+    h.upd = u;
+    h.step += 1;
+}
+
+static bool host_collides(const irbc_params &p, const HostRep &h) { return h.colsteps + 1 == p.csteps; }
+static bool host_llist(const irbc_params &p, const HostRep &h) { return h.upd + 1 == p.uplist; }
+
+extern "C" {
+
+const char *irbc_last_error(void) { return g_err.c_str(); }
+
+int64_t irbc_estimate_bytes(const irbc_replica_spec *specs, int n)
+{
+    if (!specs || n <= 0) return -1;
+    Sizes z = plan(specs, n, nullptr, nullptr, nullptr, nullptr, nullptr);
+    return bytes_of(z, n);
+}
+
+int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **out)
+{
+    if (!specs || n <= 0 || !out) return fail(IRBC_ERR_ARG, "irbc_create: bad arguments");
+    *out = nullptr;
+    for (int r = 0; r < n; ++r) {
+        const irbc_replica_spec &s = specs[r];
+        const irbc_params &p = s.params;
+        if (p.Lx < 8 || p.Ly < 8 || p.Lz < 4 || s.n_solvent <= 0 || s.n_solvent >= (1ll << 31) ||
+            s.n_vertices < 4 || s.n_edges < 6 || s.n_triangles < 4 || p.csteps < 1 || p.uplist < 1 ||
+            p.dtmd <= 0 || p.lig_spacing <= 0)
+            return fail(IRBC_ERR_ARG, "irbc_create: replica %d has invalid sizes or parameters", r);
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(IRBC_ERR_NODEVICE, "irbc_create: no HIP device");
+    if (device < 0 || device >= ndev) return fail(IRBC_ERR_ARG, "irbc_create: no device %d", device);
+    HIPCHK(hipSetDevice(device));
+    irbc_ctx *c = new irbc_ctx();
+    c->device = device;
+    c->nrep = n;
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(IRBC_ERR_HIP, "hipStreamCreate failed");
+    }
+    std::vector<ScanBlk> blks;
+    std::vector<int32_t> rfb, trep, ctrep;
+    c->z = plan(specs, n, &c->rep, &blks, &rfb, &trep, &ctrep);
+    c->hr.resize(n);
+    const Sizes &z = c->z;
+    int rc = IRBC_OK;
+#define A(p, n_)                                \
+    if (rc == IRBC_OK) rc = c->alloc(p, (n_));
+    A(c->d_rep, n) A(c->d_tile_rep, z.tiles) A(c->d_ctile_rep, z.ctiles)
+    A(c->d_rep_first_blk, n + 1) A(c->d_blk, z.scanblk) A(c->bsum, z.scanblk)
+    A(c->xa, z.slots) A(c->ya, z.slots) A(c->za, z.slots) A(c->vxa, z.slots) A(c->vya, z.slots) A(c->vza, z.slots)
+    A(c->xb, z.slots) A(c->yb, z.slots) A(c->zb, z.slots) A(c->vxb, z.slots) A(c->vyb, z.slots) A(c->vzb, z.slots)
+    A(c->ida, z.slots) A(c->idb, z.slots) A(c->skey, z.slots) A(c->srank, z.slots) A(c->skey2, z.slots)
+    A(c->ccount, z.cells) A(c->cstart, z.cells1) A(c->vhead, z.cells) A(c->lloff, z.cells1)
+    A(c->llcount, z.cells) A(c->llent, z.llent) A(c->ctab, z.cells * 8)
+    A(c->diag.vcm, 3 * z.cells) A(c->diag.ncc, z.cells) A(c->diag.rand1, z.cells)
+    A(c->diag.rand2, z.cells) A(c->diag.dvv, z.cells) A(c->diag.the, z.cells) A(c->diag.nc, z.cells)
+    A(c->cell_u, 2 * z.cells) A(c->wall_z, 3 * z.cells) A(c->mem_z, 3 * z.cells)
+    A(c->mx, z.nv) A(c->my, z.nv) A(c->mz, z.nv) A(c->mvx, z.nv) A(c->mvy, z.nv) A(c->mvz, z.nv)
+    A(c->mfx, z.nv) A(c->mfy, z.nv) A(c->mfz, z.nv) A(c->poff, z.nv) A(c->pon, z.nv)
+    A(c->mix, z.nv) A(c->miy, z.nv) A(c->mpid, z.nv) A(c->bondi, z.nv) A(c->bondl, z.nv)
+    A(c->knob, z.nv) A(c->vnext, z.nv)
+    A(c->e1, z.ne) A(c->e2, z.ne) A(c->es1, z.ne) A(c->es2, z.ne) A(c->iedge, z.ne)
+    A(c->t1, z.nt) A(c->t2, z.nt) A(c->t3, z.nt)
+    A(c->l0, z.ne) A(c->kappap, z.ne) A(c->tarea, z.nt) A(c->tri_area, z.nt) A(c->wca, 3 * z.nv)
+    A(c->slot, 3 * z.scr) A(c->inc_off, z.nv + n) A(c->inc_ent, z.inc)
+    A(c->ncand, z.nv) A(c->cand, (int64_t)z.nv * IRBC_MAX_CAND) A(c->draw_off, z.nv)
+    A(c->flag, z.nv) A(c->list, z.nv)
+    A(c->ligi, z.nlig) A(c->ligx, z.nligx) A(c->ligy, z.nligy)
+    A(c->bond_u, (int64_t)z.nv * IRBC_MAX_CAND)
+#undef A
+    int64_t maxnp = 0;
+    for (int r = 0; r < n; ++r) maxnp = std::max<int64_t>(maxnp, specs[r].n_solvent);
+    if (rc == IRBC_OK) rc = c->alloc(c->stage_f, 6 * maxnp);
+    if (rc == IRBC_OK) rc = c->alloc(c->stage_i, 2 * maxnp);
+    if (rc != IRBC_OK) {
+        std::string keep = g_err;
+        irbc_destroy(c);
+        g_err = keep;
+        return rc;
+    }
+    /* vertex-list heads start empty */
+    hipMemsetAsync(c->vhead, 0xFF, (size_t)z.cells * 4, c->st);
+    std::vector<double> lx, ly;
+    for (int r = 0; r < n; ++r) {
+        const irbc_replica_spec &s = specs[r];
+        const irbc_params &p = s.params;
+        HostRep &h = c->hr[r];
+        h.spec = s;
+        DevRep &R = c->rep[r];
+        R.navg = p.navg; R.mass = p.mass; R.Theta = p.Theta; R.dtmd = p.dtmd;
+        R.cosalpha = p.cosalpha; R.sinalpha = p.sinalpha; R.walls = p.walls;
+        R.ramp_speed = p.ramp_speed; R.ramp_steps = p.ramp_steps;
+        R.csteps = p.csteps; R.uplist = p.uplist;
+        R.kappasa = p.kappasa; R.kappad = p.kappad; R.kappav = p.kappav; R.kappab = p.kappab;
+        R.pmass = p.pmass; R.ks = p.ks; R.dthr = p.dthr; R.FD = p.FD; R.Fg = p.Fg; R.el0 = p.el0;
+        R.elap = p.elap; R.PRmass = p.PRmass; R.paraR = p.paraR; R.onrate = p.onrate;
+        R.offrate = p.offrate; R.Pi = p.Pi;
+        R.pon = 1.0 - std::exp(-p.onrate * p.dtmd);
+        R.seed = s.seed;
+        R.cur = 0;
+        R.xa = c->xa; R.ya = c->ya; R.za = c->za; R.vxa = c->vxa; R.vya = c->vya; R.vza = c->vza;
+        R.xb = c->xb; R.yb = c->yb; R.zb = c->zb; R.vxb = c->vxb; R.vyb = c->vyb; R.vzb = c->vzb;
+        R.ida = c->ida; R.idb = c->idb;
+        top_of_step(p, 0, -1, 0, R);
+        /* ligand grid coordinates (maincode.cpp:77-93) */
+        for (double m = 0.0; m < p.Lx; m += p.lig_spacing) lx.push_back(m);
+        for (double m = 0.0; m < p.Ly; m += p.lig_spacing) ly.push_back(m);
+    }
+    HIPCHK(hipMemcpyAsync(c->d_rep, c->rep.data(), n * sizeof(DevRep), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_tile_rep, trep.data(), trep.size() * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_ctile_rep, ctrep.data(), ctrep.size() * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_rep_first_blk, rfb.data(), rfb.size() * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->d_blk, blks.data(), blks.size() * sizeof(ScanBlk), hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->ligx, lx.data(), lx.size() * 8, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipMemcpyAsync(c->ligy, ly.data(), ly.size() * 8, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    *out = c;
+    return IRBC_OK;
+}
+
+int irbc_destroy(irbc_ctx *c)
+{
+    if (!c) return IRBC_OK;
+    if (c->st) hipStreamSynchronize(c->st);
+    for (void *p : c->allocs) hipFree(p);
+    for (int k = 0; k < 3; ++k)
+        for (auto &pr : c->pend[k]) {
+            hipEventDestroy(pr.first);
+            hipEventDestroy(pr.second);
+        }
+    for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->st) hipStreamDestroy(c->st);
+    delete c;
+    return IRBC_OK;
+}
+
+int irbc_device_bytes(const irbc_ctx *c, int64_t *bytes)
+{
+    if (!c || !bytes) return fail(IRBC_ERR_ARG, "irbc_device_bytes: bad arguments");
+    *bytes = c->bytes;
+    return IRBC_OK;
+}
+
+#define CHECK_R(c, r)                                                    \
+    do {                                                                 \
+        if (!(c) || (r) < 0 || (r) >= (c)->nrep)                         \
+            return fail(IRBC_ERR_ARG, "%s: bad context or replica", __func__); \
+    } while (0)
+
+int irbc_sync(irbc_ctx *c)
+{
+    if (!c) return fail(IRBC_ERR_ARG, "irbc_sync: null context");
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(c->rep.data(), c->d_rep, c->nrep * sizeof(DevRep), hipMemcpyDeviceToHost));
+    for (int r = 0; r < c->nrep; ++r)
+        if (c->rep[r].status)
+            return fail(IRBC_ERR_DEVICE, "replica %d raised device status 0x%x", r, c->rep[r].status);
+    return IRBC_OK;
+}
+
+/* ---------------- scenario setup ---------------- */
+
+int irbc_set_mesh(irbc_ctx *c, int r, const double *xyz, const int32_t *edges, const int32_t *tris)
+{
+    CHECK_R(c, r);
+    if (!xyz || !edges || !tris) return fail(IRBC_ERR_ARG, "irbc_set_mesh: null array");
+    HostRep &h = c->hr[r];
+    const irbc_params &p = h.spec.params;
+    const int nv = h.spec.n_vertices, ne = h.spec.n_edges, nt = h.spec.n_triangles;
+    for (int i = 0; i < 2 * ne; ++i)
+        if (edges[i] < 0 || edges[i] >= nv) return fail(IRBC_ERR_ARG, "irbc_set_mesh: edge index out of range");
+    for (int i = 0; i < 3 * nt; ++i)
+        if (tris[i] < 0 || tris[i] >= nv) return fail(IRBC_ERR_ARG, "irbc_set_mesh: triangle index out of range");
+    int rc = c->pull(r);
+    if (rc) return rc;
+    DevRep &R = c->rep[r];
+    std::vector<double> x(nv), y(nv), zz(nv);
+    for (int i = 0; i < nv; ++i) {
+        x[i] = xyz[3 * i];
+        y[i] = xyz[3 * i + 1];
+        zz[i] = xyz[3 * i + 2];
+    }
+    h.e1.resize(ne); h.e2.resize(ne); h.t1.resize(nt); h.t2.resize(nt); h.t3.resize(nt);
+    for (int i = 0; i < ne; ++i) { h.e1[i] = edges[2 * i]; h.e2[i] = edges[2 * i + 1]; }
+    for (int i = 0; i < nt; ++i) { h.t1[i] = tris[3 * i]; h.t2[i] = tris[3 * i + 1]; h.t3[i] = tris[3 * i + 2]; }
+    /* rest lengths (maincode.cpp:247-250) */
+    std::vector<double> l0(ne), kp(ne), ta(nt);
+    for (int i = 0; i < ne; ++i) {
+        const int a = h.e1[i], b = h.e2[i];
+        double dx = (x[a] - x[b]) * (x[a] - x[b]), dy = (y[a] - y[b]) * (y[a] - y[b]),
+               dz = (zz[a] - zz[b]) * (zz[a] - zz[b]);
+        l0[i] = std::sqrt(dx + dy + dz);
+    }
+    /* hinge partners: third vertex of the first and of the last triangle on the edge (maincode.cpp:276-342) */
+    std::vector<int32_t> es1(ne, 0), es2(ne, 0);
+    {
+        std::vector<std::vector<int32_t>> vt(nv);
+        for (int j = 0; j < nt; ++j) {
+            vt[h.t1[j]].push_back(j);
+            vt[h.t2[j]].push_back(j);
+            vt[h.t3[j]].push_back(j);
+        }
+        for (auto &l : vt) { std::sort(l.begin(), l.end()); l.erase(std::unique(l.begin(), l.end()), l.end()); }
+        for (int i = 0; i < ne; ++i) {
+            const int a = h.e1[i], b = h.e2[i];
+            std::vector<int32_t> both;
+            std::set_intersection(vt[a].begin(), vt[a].end(), vt[b].begin(), vt[b].end(), std::back_inserter(both));
+            auto third = [&](int j, int32_t &dst) {
+                if (h.t1[j] != a && h.t1[j] != b) dst = h.t1[j];
+                if (h.t2[j] != a && h.t2[j] != b) dst = h.t2[j];
+                if (h.t3[j] != a && h.t3[j] != b) dst = h.t3[j];
+            };
+            if (!both.empty()) {
+                third(both.front(), es1[i]);
+                third(both.back(), es2[i]);
+            }
+        }
+    }
+    auto area = [&](int a, int b, int cc) {
+        double A[3] = {x[b] - x[a], y[b] - y[a], zz[b] - zz[a]};
+        double B[3] = {x[cc] - x[a], y[cc] - y[a], zz[cc] - zz[a]};
+        double e0 = (A[1] * B[2]) - (B[1] * A[2]), e1_ = (A[2] * B[0]) - (B[2] * A[0]),
+               e2_ = (A[0] * B[1]) - (B[0] * A[1]);
+        return 0.5 * std::sqrt(e0 * e0 + e1_ * e1_ + e2_ * e2_);
+    };
+    auto vol = [&](int a, int b, int cc) {
+        double Cn0 = (x[a] + x[b] + x[cc]) / 3.0, Cn1 = (y[a] + y[b] + y[cc]) / 3.0,
+               Cn2 = (zz[a] + zz[b] + zz[cc]) / 3.0;
+        double A[3] = {x[b] - x[a], y[b] - y[a], zz[b] - zz[a]};
+        double B[3] = {x[cc] - x[a], y[cc] - y[a], zz[cc] - zz[a]};
+        double D0 = (A[1] * B[2]) - (B[1] * A[2]), D1 = (A[2] * B[0]) - (B[2] * A[0]),
+               D2 = (A[0] * B[1]) - (B[0] * A[1]);
+        return (1.0 / 6.0) * ((D0 * Cn0) + (D1 * Cn1) + (D2 * Cn2));
+    };
+    /* A0, V0 (TotalVA0, Src/memfuncs.cpp:192-211) */
+    double A0 = 0, V0 = 0;
+    for (int i = 0; i < nt; ++i) {
+        A0 += area(h.t1[i], h.t2[i], h.t3[i]);
+        V0 += vol(h.t1[i], h.t2[i], h.t3[i]);
+    }
+    /* kappap (maincode.cpp:347-361) */
+    for (int i = 0; i < ne; ++i) {
+        double ll0 = l0[i], lmax = 2.5 * ll0, x0 = ll0 / lmax, om = 1.0 - x0;
+        double s1 = 1.0 / (4.0 * om * om), s2 = x0 / (2.0 * om * om * om);
+        double s4 = (1.732 / (4.0 * p.elap * lmax * x0));
+        double sh = s4 * s2 - s4 * s1 + s4 * 0.25;
+        kp[i] = (p.shear - sh) * 4.0 * ll0 * ll0 * ll0 / (3.0 * 1.732);
+    }
+    for (int i = 0; i < nt; ++i) ta[i] = area(h.t1[i], h.t2[i], h.t3[i]);
+    /* theta0 (maincode.cpp:207-209) */
+    double numer = 1.73205080757 * ((double)nv - 2.0) - 5.0 * p.Pi;
+    double denom = 1.73205080757 * ((double)nv - 2.0) - 3.0 * p.Pi;
+    double theta0 = 180.0 * std::acos(numer / denom) / p.Pi;
+    /* centre of mass and hinge orientation (Src/memfuncs.cpp:55-74, 533-588) */
+    double rr[3] = {0, 0, 0};
+    for (int i = 0; i < nv; ++i) { rr[0] += x[i] + 0 * (double)p.Lx; rr[1] += y[i] + 0 * (double)p.Ly; rr[2] += zz[i]; }
+    double cen[3] = {std::fmod(rr[0] / (double)nv, (double)p.Lx), std::fmod(rr[1] / (double)nv, (double)p.Ly),
+                     std::fmod(rr[2] / (double)nv, (double)p.Lz)};
+    std::vector<int32_t> ied(ne, 1);
+    for (int i = 0; i < ne; ++i) {
+        const int l2 = h.e1[i], l3 = h.e2[i], l1 = es1[i], l4 = es2[i];
+        double P1[3] = {x[l1], y[l1], zz[l1]}, P2[3] = {x[l2], y[l2], zz[l2]};
+        double P3[3] = {x[l3], y[l3], zz[l3]}, P4[3] = {x[l4], y[l4], zz[l4]};
+        double a21[3], a31[3], a24[3], a34[3], tt1[3], tt2[3];
+        for (int d = 0; d < 3; ++d) {
+            a21[d] = P2[d] - P1[d]; a31[d] = P3[d] - P1[d];
+            a24[d] = P2[d] - P4[d]; a34[d] = P3[d] - P4[d];
+            tt1[d] = ((P1[d] + P2[d] + P3[d]) / 3.0) - cen[d];
+            tt2[d] = ((P4[d] + P2[d] + P3[d]) / 3.0) - cen[d];
+        }
+        double eta[3] = {(a21[1] * a31[2]) - (a31[1] * a21[2]), (a21[2] * a31[0]) - (a31[2] * a21[0]),
+                         (a21[0] * a31[1]) - (a31[0] * a21[1])};
+        double zeta[3] = {(a34[1] * a24[2]) - (a24[1] * a34[2]), (a34[2] * a24[0]) - (a24[2] * a34[0]),
+                          (a34[0] * a24[1]) - (a24[0] * a34[1])};
+        double che1 = eta[0] * tt1[0] + eta[1] * tt1[1] + eta[2] * tt1[2];
+        double che2 = zeta[0] * tt2[0] + zeta[1] * tt2[1] + zeta[2] * tt2[2];
+        ied[i] = (che1 < 0 && che2 < 0) ? -1 : 1;
+    }
+    /* per-vertex force-slot incidence, in the reference's summation order */
+    std::vector<std::vector<int32_t>> inc(nv);
+    for (int e = 0; e < ne; ++e) {
+        inc[h.e1[e]].push_back(4 * e + 0);
+        inc[h.e1[e]].push_back(4 * e + 2);
+        inc[h.e2[e]].push_back(4 * e + 1);
+        inc[h.e2[e]].push_back(4 * e + 3);
+    }
+    const int ST = 4 * ne, SH = 4 * ne + 9 * nt;
+    for (int t = 0; t < nt; ++t) {
+        const int vv[3] = {h.t1[t], h.t2[t], h.t3[t]};
+        for (int a = 0; a < 3; ++a)
+            for (int m = 0; m < 3; ++m) inc[vv[a]].push_back(ST + 9 * t + 3 * a + m);
+    }
+    for (int e = 0; e < ne; ++e) {
+        int l2 = h.e1[e], l3 = h.e2[e];
+        if (ied[e] == -1) std::swap(l2, l3);
+        inc[es1[e]].push_back(SH + 4 * e + 0);
+        inc[l2].push_back(SH + 4 * e + 1);
+        inc[l3].push_back(SH + 4 * e + 2);
+        inc[es2[e]].push_back(SH + 4 * e + 3);
+    }
+    std::vector<int32_t> ioff(nv + 1, 0), ient;
+    for (int v = 0; v < nv; ++v) {
+        ioff[v + 1] = ioff[v] + (int32_t)inc[v].size();
+        ient.insert(ient.end(), inc[v].begin(), inc[v].end());
+    }
+    if ((int64_t)ient.size() > inc_bound(h.spec)) return fail(IRBC_ERR_ARG, "irbc_set_mesh: incidence overflow");
+    R.Sarea0 = A0;
+    R.Volume0 = V0;
+    R.theta0 = theta0;
+    R.cos_t0 = std::cos(theta0 * p.Pi / 180.0);
+    R.sin_t0 = std::sin(theta0 * p.Pi / 180.0);
+    hipStream_t s = c->st;
+    std::vector<double> zero(nv, 0.0);
+    std::vector<int32_t> izero(nv, 0), rno(nv, IRBC_RANDNO);
+    HIPCHK(hipMemcpyAsync(c->mx + R.voff, x.data(), nv * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->my + R.voff, y.data(), nv * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->mz + R.voff, zz.data(), nv * 8, hipMemcpyHostToDevice, s));
+    double *zs[] = {c->mvx, c->mvy, c->mvz, c->mfx, c->mfy, c->mfz, c->poff, c->pon};
+    for (double *d : zs) HIPCHK(hipMemcpyAsync(d + R.voff, zero.data(), nv * 8, hipMemcpyHostToDevice, s));
+    int32_t *is[] = {c->mix, c->miy, c->mpid, c->bondi, c->knob, c->ncand};
+    for (int32_t *d : is) HIPCHK(hipMemcpyAsync(d + R.voff, izero.data(), nv * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->bondl + R.voff, rno.data(), nv * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->e1 + R.eoff, h.e1.data(), ne * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->e2 + R.eoff, h.e2.data(), ne * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->es1 + R.eoff, es1.data(), ne * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->es2 + R.eoff, es2.data(), ne * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->iedge + R.eoff, ied.data(), ne * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->l0 + R.eoff, l0.data(), ne * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->kappap + R.eoff, kp.data(), ne * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->t1 + R.toff, h.t1.data(), nt * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->t2 + R.toff, h.t2.data(), nt * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->t3 + R.toff, h.t3.data(), nt * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->tarea + R.toff, ta.data(), nt * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->inc_off + R.inc_off, ioff.data(), (nv + 1) * 4, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(c->inc_ent + R.inc_ent_off, ient.data(), ient.size() * 4, hipMemcpyHostToDevice, s));
+    h.mesh = true;
+    return c->push(r);
+}
+
+int irbc_set_vertices(irbc_ctx *c, int r, const double *x, const double *y, const double *z,
+                      const double *vx, const double *vy, const double *vz, const double *fx,
+                      const double *fy, const double *fz, const int32_t *ix, const int32_t *iy,
+                      const int32_t *pid, const int32_t *bondi, const int32_t *bondl,
+                      const int32_t *knob)
+{
+    CHECK_R(c, r);
+    const DevRep &R = c->rep[r];
+    const size_t n = c->hr[r].spec.n_vertices;
+    const double *ds[] = {x, y, z, vx, vy, vz, fx, fy, fz};
+    double *dd[] = {c->mx, c->my, c->mz, c->mvx, c->mvy, c->mvz, c->mfx, c->mfy, c->mfz};
+    for (int k = 0; k < 9; ++k)
+        if (ds[k]) HIPCHK(hipMemcpyAsync(dd[k] + R.voff, ds[k], n * 8, hipMemcpyHostToDevice, c->st));
+    const int32_t *is[] = {ix, iy, pid, bondi, bondl, knob};
+    int32_t *id[] = {c->mix, c->miy, c->mpid, c->bondi, c->bondl, c->knob};
+    for (int k = 0; k < 6; ++k)
+        if (is[k]) HIPCHK(hipMemcpyAsync(id[k] + R.voff, is[k], n * 4, hipMemcpyHostToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    return IRBC_OK;
+}
+
+int irbc_get_vertices(irbc_ctx *c, int r, double *x, double *y, double *z, double *vx, double *vy,
+                      double *vz, double *fx, double *fy, double *fz, double *poff, double *pon,
+                      int32_t *ix, int32_t *iy, int32_t *pid, int32_t *bondi, int32_t *bondl,
+                      int32_t *knob)
+{
+    CHECK_R(c, r);
+    const DevRep &R = c->rep[r];
+    const size_t n = c->hr[r].spec.n_vertices;
+    HIPCHK(hipStreamSynchronize(c->st));
+    double *dd[] = {x, y, z, vx, vy, vz, fx, fy, fz, poff, pon};
+    const double *ds[] = {c->mx, c->my, c->mz, c->mvx, c->mvy, c->mvz, c->mfx, c->mfy, c->mfz, c->poff, c->pon};
+    for (int k = 0; k < 11; ++k)
+        if (dd[k]) HIPCHK(hipMemcpy(dd[k], ds[k] + R.voff, n * 8, hipMemcpyDeviceToHost));
+    int32_t *id[] = {ix, iy, pid, bondi, bondl, knob};
+    const int32_t *is[] = {c->mix, c->miy, c->mpid, c->bondi, c->bondl, c->knob};
+    for (int k = 0; k < 6; ++k)
+        if (id[k]) HIPCHK(hipMemcpy(id[k], is[k] + R.voff, n * 4, hipMemcpyDeviceToHost));
+    return IRBC_OK;
+}
+
+int irbc_set_solvent(irbc_ctx *c, int r, int64_t n, const double *x, const double *y,
+                     const double *z, const double *vx, const double *vy, const double *vz,
+                     const int32_t *indio)
+{
+    CHECK_R(c, r);
+    if (n != c->hr[r].spec.n_solvent || !x || !y || !z || !vx || !vy || !vz)
+        return fail(IRBC_ERR_ARG, "irbc_set_solvent: expected %lld particles", (long long)c->hr[r].spec.n_solvent);
+    int rc = c->pull(r);
+    if (rc) return rc;
+    DevRep &R = c->rep[r];
+    std::vector<float> f(n);
+    const double *src[] = {x, y, z, vx, vy, vz};
+    float *dst[] = {c->xa, c->ya, c->za, c->vxa, c->vya, c->vza};
+    for (int k = 0; k < 6; ++k) {
+        for (int64_t i = 0; i < n; ++i) f[i] = (float)src[k][i];
+        HIPCHK(hipMemcpy(dst[k] + R.poff, f.data(), n * 4, hipMemcpyHostToDevice));
+    }
+    std::vector<uint32_t> id(n);
+    for (int64_t i = 0; i < n; ++i) id[i] = (uint32_t)i | ((indio ? (uint32_t)(indio[i] != 0) : 1u) << 31);
+    HIPCHK(hipMemcpy(c->ida + R.poff, id.data(), n * 4, hipMemcpyHostToDevice));
+    std::vector<int32_t> zero(n, 0);
+    HIPCHK(hipMemcpy(c->skey2 + R.poff, zero.data(), n * 4, hipMemcpyHostToDevice));
+    R.cur = 0;
+    R.rotate_pending = 0;
+    c->hr[r].solvent = true;
+    return c->push(r);
+}
+
+int irbc_get_solvent(irbc_ctx *c, int r, double *x, double *y, double *z, double *vx, double *vy,
+                     double *vz, int32_t *key, int32_t *indio)
+{
+    CHECK_R(c, r);
+    const int64_t n = c->hr[r].spec.n_solvent;
+    HIPCHK(hipStreamSynchronize(c->st));
+    k_unpermute<<<(unsigned)((n + IRBC_WG - 1) / IRBC_WG), IRBC_WG, 0, c->st>>>(c->d_rep, r, c->skey2,
+                                                                               c->stage_f, c->stage_i);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(c->st));
+    std::vector<float> f(6 * n);
+    std::vector<int32_t> iv(2 * n);
+    HIPCHK(hipMemcpy(f.data(), c->stage_f, 6 * n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(iv.data(), c->stage_i, 2 * n * 4, hipMemcpyDeviceToHost));
+    double *dd[] = {x, y, z, vx, vy, vz};
+    for (int k = 0; k < 6; ++k)
+        if (dd[k])
+            for (int64_t i = 0; i < n; ++i) dd[k][i] = (double)f[k * n + i];
+    if (key) std::memcpy(key, iv.data(), n * 4);
+    if (indio) std::memcpy(indio, iv.data() + n, n * 4);
+    return IRBC_OK;
+}
+
+int irbc_set_parasite(irbc_ctx *c, int r, const double *d, const int32_t *i)
+{
+    CHECK_R(c, r);
+    if (!d || !i) return fail(IRBC_ERR_ARG, "irbc_set_parasite: null array");
+    int rc = c->pull(r);
+    if (rc) return rc;
+    DevRep &R = c->rep[r];
+    for (int k = 0; k < 3; ++k) {
+        R.para[k] = d[k];
+        R.paraV[k] = d[3 + k];
+        R.paraft[k] = d[6 + k];
+        R.parafdt[k] = d[9 + k];
+    }
+    R.paradx = i[0];
+    R.parady = i[1];
+    R.paraid = i[2];
+    return c->push(r);
+}
+
+int irbc_get_parasite(irbc_ctx *c, int r, double *d, int32_t *i)
+{
+    CHECK_R(c, r);
+    int rc = c->pull(r);
+    if (rc) return rc;
+    const DevRep &R = c->rep[r];
+    for (int k = 0; k < 3; ++k) {
+        d[k] = R.para[k];
+        d[3 + k] = R.paraV[k];
+        d[6 + k] = R.paraft[k];
+        d[9 + k] = R.parafdt[k];
+    }
+    i[0] = R.paradx;
+    i[1] = R.parady;
+    i[2] = R.paraid;
+    return IRBC_OK;
+}
+
+int64_t irbc_n_ligands(const irbc_ctx *c, int r)
+{
+    if (!c || r < 0 || r >= c->nrep) return IRBC_ERR_ARG;
+    return c->rep[r].nlig;
+}
+
+int irbc_set_ligands(irbc_ctx *c, int r, const int32_t *ligi)
+{
+    CHECK_R(c, r);
+    HIPCHK(hipMemcpy(c->ligi + c->rep[r].loff, ligi, (size_t)c->rep[r].nlig * 4, hipMemcpyHostToDevice));
+    return IRBC_OK;
+}
+
+int irbc_get_ligands(irbc_ctx *c, int r, int32_t *ligi)
+{
+    CHECK_R(c, r);
+    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(hipMemcpy(ligi, c->ligi + c->rep[r].loff, (size_t)c->rep[r].nlig * 4, hipMemcpyDeviceToHost));
+    return IRBC_OK;
+}
+
+int irbc_get_globals(irbc_ctx *c, int r, double *g)
+{
+    CHECK_R(c, r);
+    int rc = c->pull(r);
+    if (rc) return rc;
+    const DevRep &R = c->rep[r];
+    g[0] = R.Sarea;
+    g[1] = R.Volume;
+    g[2] = R.Sarea0;
+    g[3] = R.Volume0;
+    g[4] = R.theta0;
+    g[5] = R.wallspeed;
+    g[6] = R.status;
+    g[7] = R.bond_draws;
+    g[8] = (double)R.step;
+    g[9] = R.nbonds;
+    return IRBC_OK;
+}
+
+int irbc_set_counters(irbc_ctx *c, int r, int64_t step, int32_t colsteps, int32_t upd)
+{
+    CHECK_R(c, r);
+    int rc = c->pull(r);
+    if (rc) return rc;
+    HostRep &h = c->hr[r];
+    h.step = step;
+    h.colsteps = colsteps;
+    h.upd = upd;
+    top_of_step(h.spec.params, step, colsteps, upd, c->rep[r]);
+    return c->push(r);
+}
+
+int irbc_get_counters(irbc_ctx *c, int r, int64_t *step, int32_t *colsteps, int32_t *upd)
+{
+    CHECK_R(c, r);
+    *step = c->hr[r].step;
+    *colsteps = c->hr[r].colsteps;
+    *upd = c->hr[r].upd;
+    return IRBC_OK;
+}
+
+int irbc_set_wallspeed(irbc_ctx *c, int r, double w)
+{
+    CHECK_R(c, r);
+    int rc = c->pull(r);
+    if (rc) return rc;
+    c->rep[r].wallspeed = w;
+    return c->push(r);
+}
+
+int irbc_set_seed(irbc_ctx *c, int r, uint64_t seed)
+{
+    CHECK_R(c, r);
+    int rc = c->pull(r);
+    if (rc) return rc;
+    c->rep[r].seed = seed;
+    c->hr[r].spec.seed = seed;
+    return c->push(r);
+}
+
+int irbc_use_tapes(irbc_ctx *c, int r, int on)
+{
+    CHECK_R(c, r);
+    int rc = c->pull(r);
+    if (rc) return rc;
+    c->rep[r].tapes = on ? 1 : 0;
+    return c->push(r);
+}
+
+int irbc_set_collision_tape(irbc_ctx *c, int r, const double *shift_u, const double *cell_u,
+                            const double *wall_z, const double *mem_z)
+{
+    CHECK_R(c, r);
+    if (!shift_u || !cell_u || !wall_z || !mem_z) return fail(IRBC_ERR_ARG, "irbc_set_collision_tape: null tape");
+    int rc = c->pull(r);
+    if (rc) return rc;
+    DevRep &R = c->rep[r];
+    const size_t nc = R.ncell;
+    HIPCHK(hipMemcpy(c->cell_u + 2 * R.coff, cell_u, 2 * nc * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->wall_z + 3 * R.coff, wall_z, 3 * nc * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->mem_z + 3 * R.coff, mem_z, 3 * nc * 8, hipMemcpyHostToDevice));
+    for (int k = 0; k < 3; ++k) R.shift[k] = 1.0 * shift_u[k];
+    R.tapes = 1;
+    return c->push(r);
+}
+
+int irbc_set_bond_tape(irbc_ctx *c, int r, const double *u, int64_t n)
+{
+    CHECK_R(c, r);
+    const DevRep &R = c->rep[r];
+    if (n < 0 || n > (int64_t)R.nv * IRBC_MAX_CAND) return fail(IRBC_ERR_ARG, "irbc_set_bond_tape: too many draws");
+    if (n > 0) HIPCHK(hipMemcpy(c->bond_u + (int64_t)R.voff * IRBC_MAX_CAND, u, n * 8, hipMemcpyHostToDevice));
+    return IRBC_OK;
+}
+
+/* ---------------- stages ---------------- */
+
+static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t *out)
+{
+    const unsigned nb = (unsigned)c->z.scanblk;
+    k_scan_reduce<<<nb, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_blk, staged, which, in, c->bsum);
+    k_scan_top<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_blk, c->d_rep_first_blk, staged, which, c->bsum);
+    k_scan_apply<<<nb, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_blk, staged, which, in, c->bsum, out);
+    LAUNCHCHK();
+    return IRBC_OK;
+}
+
+static int launch_collision(irbc_ctx *c, int staged, int diag)
+{
+    const unsigned T = (unsigned)c->z.tiles, CT = (unsigned)c->z.ctiles;
+    if (staged) k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 0);
+    k_bin<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, staged, c->skey, c->srank, c->ccount);
+    int rc = launch_scan(c, staged, 0, c->ccount, c->cstart);
+    if (rc) return rc;
+    k_scatter<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, staged, c->skey, c->srank, c->cstart, c->skey2);
+    MemDev M = c->mem();
+    k_vbin<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, staged, M, c->vhead, c->vnext);
+    k_cell<<<CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, staged, c->cstart, c->vhead, c->vnext,
+                                      c->mvx, c->mvy, c->mvz, c->cell_u, c->wall_z, c->mem_z, c->ctab,
+                                      diag, c->diag);
+    k_vrot<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, staged, M, c->ctab);
+    if (staged) k_rotate<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab);
+    LAUNCHCHK();
+    return IRBC_OK;
+}
+
+static int launch_llist(irbc_ctx *c, int staged)
+{
+    int maxnt = 0;
+    for (int r = 0; r < c->nrep; ++r) maxnt = std::max(maxnt, c->rep[r].nt);
+    dim3 g((unsigned)((maxnt * 64 + IRBC_WG - 1) / IRBC_WG), (unsigned)c->nrep);
+    MemView V = c->view();
+    k_tri_count<<<g, IRBC_WG, 0, c->st>>>(c->d_rep, staged, V, c->mix, c->miy, c->llcount);
+    int rc = launch_scan(c, staged, 1, c->llcount, c->lloff);
+    if (rc) return rc;
+    k_tri_fill<<<g, IRBC_WG, 0, c->st>>>(c->d_rep, staged, V, c->mix, c->miy, c->lloff, c->llcount, c->llent);
+    k_tri_sort<<<(unsigned)c->z.ctiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, staged, c->lloff,
+                                                            c->llcount, c->llent);
+    LAUNCHCHK();
+    return IRBC_OK;
+}
+
+static int ready(irbc_ctx *c, bool need_solvent)
+{
+    for (int r = 0; r < c->nrep; ++r) {
+        if (!c->hr[r].mesh) return fail(IRBC_ERR_STATE, "replica %d has no mesh (irbc_set_mesh)", r);
+        if (need_solvent && !c->hr[r].solvent)
+            return fail(IRBC_ERR_STATE, "replica %d has no solvent (irbc_set_solvent)", r);
+    }
+    return IRBC_OK;
+}
+
+int irbc_collision_step(irbc_ctx *c)
+{
+    if (!c) return fail(IRBC_ERR_ARG, "null context");
+    int rc = ready(c, true);
+    if (rc) return rc;
+    rc = launch_collision(c, 1, 1);
+    if (rc) return rc;
+    return irbc_sync(c);
+}
+
+int irbc_llist_update(irbc_ctx *c)
+{
+    if (!c) return fail(IRBC_ERR_ARG, "null context");
+    int rc = ready(c, false);
+    if (rc) return rc;
+    rc = launch_llist(c, 1);
+    if (rc) return rc;
+    return irbc_sync(c);
+}
+
+int irbc_stream(irbc_ctx *c)
+{
+    if (!c) return fail(IRBC_ERR_ARG, "null context");
+    int rc = ready(c, true);
+    if (rc) return rc;
+    k_stream<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab,
+                                                         c->lloff, c->llent, c->view());
+    LAUNCHCHK();
+    return irbc_sync(c);
+}
+
+int irbc_classify_solvent(irbc_ctx *c)
+{
+    if (!c) return fail(IRBC_ERR_ARG, "null context");
+    int rc = ready(c, true);
+    if (rc) return rc;
+    k_classify<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->view(), c->mix, c->miy);
+    LAUNCHCHK();
+    return irbc_sync(c);
+}
+
+int irbc_membrane_phases(irbc_ctx *c, int phases)
+{
+    if (!c) return fail(IRBC_ERR_ARG, "null context");
+    int rc = ready(c, false);
+    if (rc) return rc;
+    k_membrane<<<c->nrep, IRBC_MEM_WG, 0, c->st>>>(c->d_rep, phases, c->mem());
+    LAUNCHCHK();
+    if (phases & PH_COUNTERS)
+        for (int r = 0; r < c->nrep; ++r) advance_host(c->hr[r].spec.params, c->hr[r]);
+    return irbc_sync(c);
+}
+
+int irbc_membrane_va(irbc_ctx *c, int r, double *area, double *volume)
+{
+    CHECK_R(c, r);
+    int rc = irbc_membrane_phases(c, PH_VA);
+    if (rc) return rc;
+    if (area) *area = c->rep[r].Sarea;
+    if (volume) *volume = c->rep[r].Volume;
+    return IRBC_OK;
+}
+
+int irbc_membrane_forces(irbc_ctx *c) { return irbc_membrane_phases(c, PH_VA | PH_FORCES | PH_EXTERNAL); }
+
+int irbc_bond_step(irbc_ctx *c) { return irbc_membrane_phases(c, PH_BONDS); }
+
+int irbc_get_cells(irbc_ctx *c, int r, double *vcm, int32_t *nc, double *ncc, double *rand1,
+                   double *rand2, double *dvv, double *the)
+{
+    CHECK_R(c, r);
+    const DevRep &R = c->rep[r];
+    const size_t n = R.ncell;
+    HIPCHK(hipStreamSynchronize(c->st));
+    if (vcm) HIPCHK(hipMemcpy(vcm, c->diag.vcm + 3 * R.coff, 3 * n * 8, hipMemcpyDeviceToHost));
+    if (nc) HIPCHK(hipMemcpy(nc, c->diag.nc + R.coff, n * 4, hipMemcpyDeviceToHost));
+    double *dd[] = {ncc, rand1, rand2, dvv, the};
+    double *ds[] = {c->diag.ncc, c->diag.rand1, c->diag.rand2, c->diag.dvv, c->diag.the};
+    for (int k = 0; k < 5; ++k)
+        if (dd[k]) HIPCHK(hipMemcpy(dd[k], ds[k] + R.coff, n * 8, hipMemcpyDeviceToHost));
+    return IRBC_OK;
+}
+
+int64_t irbc_get_llist(irbc_ctx *c, int r, int64_t *off, int32_t *ent, int64_t cap)
+{
+    CHECK_R(c, r);
+    const DevRep &R = c->rep[r];
+    const size_t n = R.ncell;
+    HIPCHK(hipStreamSynchronize(c->st));
+    std::vector<int32_t> o(n + 1);
+    HIPCHK(hipMemcpy(o.data(), c->lloff + R.coff1, (n + 1) * 4, hipMemcpyDeviceToHost));
+    const int64_t tot = o[n];
+    if (off)
+        for (size_t i = 0; i <= n; ++i) off[i] = o[i];
+    if (ent && tot > 0) HIPCHK(hipMemcpy(ent, c->llent + R.llent_off, std::min(tot, cap) * 4, hipMemcpyDeviceToHost));
+    return tot;
+}
+
+int64_t irbc_get_pairs(irbc_ctx *c, int r, int32_t *cnt, int32_t *lig, int64_t cap)
+{
+    CHECK_R(c, r);
+    const DevRep &R = c->rep[r];
+    const int nv = R.nv;
+    HIPCHK(hipStreamSynchronize(c->st));
+    std::vector<int32_t> nc(nv), cd((size_t)nv * IRBC_MAX_CAND);
+    HIPCHK(hipMemcpy(nc.data(), c->ncand + R.voff, nv * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cd.data(), c->cand + (int64_t)R.voff * IRBC_MAX_CAND, cd.size() * 4, hipMemcpyDeviceToHost));
+    int64_t t = 0;
+    for (int v = 0; v < nv; ++v) {
+        if (cnt) cnt[v] = nc[v];
+        for (int k = 0; k < nc[v]; ++k) {
+            if (lig && t < cap) lig[t] = cd[(size_t)v * IRBC_MAX_CAND + k];
+            ++t;
+        }
+    }
+    return t;
+}
+
+/* ---------------- the time loop ---------------- */
+
+int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
+{
+    if (!c || nsteps < 0) return fail(IRBC_ERR_ARG, "irbc_md_steps: bad arguments");
+    int rc = ready(c, true);
+    if (rc) return rc;
+    const unsigned T = (unsigned)c->z.tiles;
+    const MemView V = c->view();
+    const MemDev M = c->mem();
+    k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 1);
+    for (int64_t s = 0; s < nsteps; ++s) {
+        bool col = false, ll = false;
+        for (int r = 0; r < c->nrep; ++r) {
+            col |= host_collides(c->hr[r].spec.params, c->hr[r]);
+            ll |= host_llist(c->hr[r].spec.params, c->hr[r]);
+        }
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        const bool rec = c->prof && (c->prof_step++ % c->prof_every) == 0;
+        if (col) {
+            if (rec) { e0 = c->event(); hipEventRecord(e0, c->st); }
+            rc = launch_collision(c, 0, c->diag_on ? 1 : 0);
+            if (rc) return rc;
+            if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[1].push_back({e0, e1}); }
+        }
+        if (ll) {
+            rc = launch_llist(c, 0);
+            if (rc) return rc;
+        }
+        if (rec) { e0 = c->event(); hipEventRecord(e0, c->st); }
+        k_stream<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab, c->lloff, c->llent, V);
+        if (rec) {
+            e1 = c->event();
+            hipEventRecord(e1, c->st);
+            c->pend[0].push_back({e0, e1});
+            e0 = c->event();
+            hipEventRecord(e0, c->st);
+        }
+        k_membrane<<<c->nrep, IRBC_MEM_WG, 0, c->st>>>(c->d_rep, PH_ALL, M);
+        if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[2].push_back({e0, e1}); }
+        LAUNCHCHK();
+        for (int r = 0; r < c->nrep; ++r) advance_host(c->hr[r].spec.params, c->hr[r]);
+        if (c->prof && c->pend[0].size() >= 2048) {
+            rc = c->flush_prof();
+            if (rc) return rc;
+        }
+    }
+    return IRBC_OK;
+}
+
+int irbc_set_diagnostics(irbc_ctx *c, int on)
+{
+    if (!c) return fail(IRBC_ERR_ARG, "null context");
+    c->diag_on = on != 0;
+    return IRBC_OK;
+}
+
+int irbc_profile(irbc_ctx *c, int on)
+{
+    if (!c) return fail(IRBC_ERR_ARG, "null context");
+    int rc = c->flush_prof();
+    if (rc) return rc;
+    for (int k = 0; k < 3; ++k) {
+        c->acc_ms[k] = 0;
+        c->acc_n[k] = 0;
+    }
+    c->prof = on > 0;
+    c->prof_every = on > 0 ? on : 0;
+    c->prof_step = 0;
+    return IRBC_OK;
+}
+
+int irbc_profile_read(irbc_ctx *c, double *t)
+{
+    if (!c || !t) return fail(IRBC_ERR_ARG, "irbc_profile_read: bad arguments");
+    int rc = c->flush_prof();
+    if (rc) return rc;
+    for (int k = 0; k < 3; ++k) {
+        t[2 * k] = c->acc_ms[k];
+        t[2 * k + 1] = (double)c->acc_n[k];
+    }
+    return IRBC_OK;
+}
+
+} /* extern "C" */

@@ -140,9 +140,12 @@ __device__ __forceinline__ void philox_n2(uint64_t seed, uint32_t stream, uint64
     z1 = r * sin(t);
 }
 
-/* the reference's bbound() while-loop wrap (Src/solfuncs.cpp:603-622) */
+/* the reference's bbound() while-loop wrap (Src/solfuncs.cpp:603-622); a
+ * coordinate more than 64 periods away (or not finite) cannot come from a
+ * valid state and maps to 0 so that the loop stays bounded */
 __device__ __forceinline__ double wrap_loop(double x, double l)
 {
+    if (!(x >= -64.0 * l && x <= 64.0 * l)) return 0.0;
     if (x > l)
         while (x > l) x -= l;
     if (x < 0)

@@ -149,6 +149,7 @@ struct orc {
     std::vector<double> mx, my, mz, mvx, mvy, mvz, fx, fy, fz, poff, pon;
     std::vector<int32_t> ix, iy, mpid, bondi, bondl, knob;
     std::vector<std::vector<std::pair<double, int>>> pairs;
+    std::vector<int32_t> prep_live; /* took the free-knob branch in the last bond_prep */
     std::vector<int32_t> e1, e2, es1, es2, iedge;
     std::vector<double> l0, kappap;
     std::vector<int32_t> t1, t2, t3;
@@ -425,6 +426,7 @@ int orc_set_mesh(orc *o, int nv, const double *xyz, int ne, const int32_t *edges
     izero(o->ix); izero(o->iy); izero(o->mpid); izero(o->bondi); izero(o->knob);
     o->bondl.assign(nv, 12345678); /* randno (define.h:37) */
     o->pairs.assign(nv, {});
+    o->prep_live.assign(nv, 0);
     for (int i = 0; i < nv; ++i) {
         o->mx[i] = xyz[3 * i];
         o->my[i] = xyz[3 * i + 1];
@@ -660,8 +662,7 @@ int64_t orc_get_pairs(const orc *o, int32_t *cnt, int32_t *lig, int64_t cap)
 {
     int64_t t = 0;
     for (int v = 0; v < o->nv; ++v) {
-        bool live = (o->bondi[v] == 0) && (o->mz[v] < o->p.dthr) && (o->knob[v] == 1);
-        int c = live ? (int)o->pairs[v].size() : 0;
+        int c = o->prep_live[v] ? (int)o->pairs[v].size() : 0;
         if (cnt) cnt[v] = c;
         for (int k = 0; k < c; ++k) {
             if (lig && t < cap) lig[t] = o->pairs[v][k].second;
@@ -1455,6 +1456,7 @@ void orc_bond_prep(orc *o)
      * for free knobs near the wall (maincode.cpp:668-723) */
     const double ks = o->p.ks, el0 = o->p.el0, dthr = o->p.dthr, dtmd = o->p.dtmd;
     for (int ii = 0; ii < o->nv; ++ii) {
+        o->prep_live[ii] = 0;
         if (o->bondi[ii] == 1) {
             int p1modx = o->ix[ii], p1mody = o->iy[ii];
             int jj = o->bondl[ii];
@@ -1476,6 +1478,7 @@ void orc_bond_prep(orc *o)
         } else if ((o->mz[ii] < dthr) && (o->bondi[ii] == 0) && (o->knob[ii] == 1)) {
             auto &pr = o->pairs[ii];
             pr.clear();
+            o->prep_live[ii] = 1;
             int c = o->mpid[ii];
             double kon = o->p.onrate;
             o->pon[ii] = 1.0 - std::exp(-kon * dtmd);
@@ -1587,11 +1590,12 @@ void orc_makeiedge(orc *o)
     }
 }
 
-void orc_sortsolvent(orc *o)
+void orc_set_wallspeed(orc *o, double w) { o->wallspeed = w; }
+
+void orc_classify(orc *o)
 {
-    /* sortsolvent_initial1 (Src/solfuncs.cpp:17-106): approximate ray parity
-     * (+x ray, triangle centroid ahead of the particle), then std::sort of
-     * the whole particle array by the flag */
+    /* the flag loop of sortsolvent_initial1 (Src/solfuncs.cpp:17-94):
+     * approximate ray parity (+x ray, triangle centroid ahead of the particle) */
     orc_tricm(o);
     int64_t ns = (int64_t)o->x.size();
     int nt = (int)o->t1.size();
@@ -1638,6 +1642,14 @@ void orc_sortsolvent(orc *o)
         }
         o->indio[i] = ((count % 2) == 0) ? 1 : 0;
     }
+}
+
+void orc_sortsolvent(orc *o)
+{
+    /* sortsolvent_initial1 (Src/solfuncs.cpp:17-106): flags, then std::sort
+     * of the whole particle array by the flag */
+    orc_classify(o);
+    int64_t ns = (int64_t)o->x.size();
     /* std::sort on the flag alone: sorting (flag, index) records with the
      * same comparator reproduces libstdc++'s permutation of the particles */
     struct Rec {

@@ -115,6 +115,8 @@ void orc_kick2(orc *o);         /* membrane + parasite second half-kick */
 void orc_centerofmass(orc *o);
 void orc_makeiedge(orc *o);
 void orc_sortsolvent(orc *o);   /* inside/outside flags + std::sort by flag */
+void orc_classify(orc *o);      /* inside/outside flags only, particle order kept */
+void orc_set_wallspeed(orc *o, double w);
 int orc_solve_p3(double *x, double a, double b, double c);
 double orc_solve(double a, double b, double c, double dtmd);
 

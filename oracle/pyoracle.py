@@ -102,6 +102,7 @@ def lib():
         L.orc_solve_p3.restype = C.c_int
         L.orc_solve_p3.argtypes = [_d, C.c_double, C.c_double, C.c_double]
         L.orc_stream.argtypes = [C.c_void_p, C.c_double]
+        L.orc_set_wallspeed.argtypes = [C.c_void_p, C.c_double]
         L.orc_set_counters.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int32]
         L.orc_set_bond_tape.argtypes = [C.c_void_p, _d, C.c_int64]
         L.orc_get_bond_tape.argtypes = [C.c_void_p, _d, C.c_int64]
@@ -322,16 +323,19 @@ class Oracle:
     def stream(self, wallspeed):
         self.L.orc_stream(self.h, float(wallspeed))
 
+    def set_wallspeed(self, w):
+        self.L.orc_set_wallspeed(self.h, float(w))
+
     def reflection(self, i):
         return self.L.orc_reflection(self.h, int(i))
 
     def __getattr__(self, name):
         # zero-argument stages: kick1, drift, totalva, totalva0, pfdt, forces,
         # external, bond_prep, bond_dynamics, kick2, centerofmass, makeiedge,
-        # sortsolvent, tricm, md_step
+        # sortsolvent, tricm, md_step, classify
         if name in ("kick1", "drift", "totalva", "totalva0", "pfdt", "forces", "external",
                     "bond_prep", "bond_dynamics", "kick2", "centerofmass", "makeiedge",
-                    "sortsolvent", "tricm", "md_step"):
+                    "sortsolvent", "tricm", "md_step", "classify"):
             f = getattr(lib(), "orc_" + name)
             return lambda: f(self.h)
         raise AttributeError(name)
