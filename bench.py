@@ -153,7 +153,8 @@ def main():
         "phases_ms_per_launch": {
             "stream": k_ms,
             "collision_sequence": prof["collision_ms"] / max(prof["collision_n"], 1),
-            "membrane": prof["membrane_ms"] / max(prof["membrane_n"], 1)},
+            "membrane": prof["membrane_ms"] / max(prof["membrane_n"], 1),
+            "stream_near": prof["near_ms"] / max(prof["near_n"], 1)},
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(b, setup)

@@ -155,9 +155,10 @@ int irbc_set_diagnostics(irbc_ctx *ctx, int on);
 /* every > 0: bracket the launches of every every-th step of irbc_md_steps
  * with events; 0: off.  Resets the accumulators. */
 int irbc_profile(irbc_ctx *ctx, int every);
-/* t[0] = total ms of timed stream launches, t[1] = their count,
+/* t[0] = total ms of timed bulk-stream launches, t[1] = their count,
  * t[2] = total ms of collision sequences, t[3] = their count,
- * t[4] = total ms of membrane launches, t[5] = their count */
+ * t[4] = total ms of membrane launches, t[5] = their count,
+ * t[6] = total ms of near-membrane stream launches, t[7] = their count */
 int irbc_profile_read(irbc_ctx *ctx, double *t);
 
 #ifdef __cplusplus

@@ -6,6 +6,7 @@ Outputs:
   irbc-code_amd/libirbc_amd.so   HIP kernels + C ABI (hipcc, gfx950)
   irbc-code_amd/irbc_main        reference-style command-line driver
   irbc-code_amd/librefnames.so   reference-named functions over the C ABI
+  irbc-code_amd/copy_probe       HBM copy-bandwidth probe (counter calibration)
   oracle/liboracle.so            CPU checker (test infrastructure)
   oracle/_ref/ref_run            harnessed reference build (only where the
                                  reference sources are present)
@@ -39,6 +40,12 @@ def build_lib():
     return out
 
 
+def build_probe():
+    """Copy-bandwidth probe used to calibrate the HBM traffic counters."""
+    run([HIPCC] + HIPFLAGS + ["-o", os.path.join(HERE, "copy_probe"),
+                              os.path.join(HERE, "csrc", "copy_probe.hip")])
+
+
 def build_host():
     inc = os.path.join(REPO, "include")
     lib = ["-L" + HERE, "-lirbc_amd", "-Wl,-rpath,$ORIGIN"]
@@ -59,6 +66,7 @@ def build_oracle():
 
 def build_all():
     build_lib()
+    build_probe()
     build_host()
     build_oracle()
 
@@ -66,4 +74,5 @@ def build_all():
 if __name__ == "__main__":
     what = sys.argv[1:] or ["all"]
     for w in what:
-        {"all": build_all, "lib": build_lib, "host": build_host, "oracle": build_oracle}[w]()
+        {"all": build_all, "lib": build_lib, "host": build_host, "oracle": build_oracle,
+         "probe": build_probe}[w]()

@@ -165,7 +165,7 @@ Sizes plan(const irbc_replica_spec *specs, int n, std::vector<DevRep> *reps,
 int64_t bytes_of(const Sizes &z, int n)
 {
     int64_t b = 0;
-    b += z.slots * (2 * 6 * 4 + 2 * 4 + 3 * 4);           /* particles x2, ids x2, keys, ranks, sorted keys */
+    b += z.slots * (2 * 6 * 4 + 2 * 4 + 4 * 4);           /* particles x2, ids x2, keys, ranks, sorted keys, queue */
     b += z.cells * (4 + 8 * 8 + 4 + 4 + 8 * 8 + 4);       /* count, ctab, vhead, llcount, diag, cursor */
     b += z.cells * 8 * 8;                                 /* tapes */
     b += z.cells1 * (4 + 4);                              /* start, lloff */
@@ -197,7 +197,7 @@ struct irbc_ctx {
     /* solvent */
     float *xa, *ya, *za, *vxa, *vya, *vza, *xb, *yb, *zb, *vxb, *vyb, *vzb;
     uint32_t *ida, *idb;
-    int32_t *skey, *srank, *skey2;
+    int32_t *skey, *srank, *skey2, *nearq;
     /* cells */
     int32_t *ccount, *cstart, *vhead, *lloff, *llcount, *llent;
     double *ctab;
@@ -216,14 +216,15 @@ struct irbc_ctx {
     float *stage_f;
     int32_t *stage_i;
     bool diag_on = false;
+    size_t mem_lds = 0; /* dynamic LDS of k_membrane: unwrapped positions, 24 B per vertex */
     /* profiling: HIP events around every prof_every-th step's launches */
     bool prof = false;
     int prof_every = 0;
     int64_t prof_step = 0;
     std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> pend[3];
-    double acc_ms[3] = {0, 0, 0};
-    int64_t acc_n[3] = {0, 0, 0};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pend[4];
+    double acc_ms[4] = {0, 0, 0, 0};
+    int64_t acc_n[4] = {0, 0, 0, 0};
 
     template <class T> int alloc(T *&p, int64_t n)
     {
@@ -288,7 +289,7 @@ struct irbc_ctx {
     int flush_prof()
     {
         HIPCHK(hipStreamSynchronize(st));
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 4; ++k) {
             for (auto &pr : pend[k]) {
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
@@ -377,7 +378,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     A(c->d_rep_first_blk, n + 1) A(c->d_blk, z.scanblk) A(c->bsum, z.scanblk)
     A(c->xa, z.slots) A(c->ya, z.slots) A(c->za, z.slots) A(c->vxa, z.slots) A(c->vya, z.slots) A(c->vza, z.slots)
     A(c->xb, z.slots) A(c->yb, z.slots) A(c->zb, z.slots) A(c->vxb, z.slots) A(c->vyb, z.slots) A(c->vzb, z.slots)
-    A(c->ida, z.slots) A(c->idb, z.slots) A(c->skey, z.slots) A(c->srank, z.slots) A(c->skey2, z.slots)
+    A(c->ida, z.slots) A(c->idb, z.slots) A(c->skey, z.slots) A(c->srank, z.slots) A(c->skey2, z.slots) A(c->nearq, z.slots)
     A(c->ccount, z.cells) A(c->cstart, z.cells1) A(c->vhead, z.cells) A(c->lloff, z.cells1)
     A(c->llcount, z.cells) A(c->llent, z.llent) A(c->ctab, z.cells * 8)
     A(c->diag.vcm, 3 * z.cells) A(c->diag.ncc, z.cells) A(c->diag.rand1, z.cells)
@@ -397,7 +398,18 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     A(c->bond_u, (int64_t)z.nv * IRBC_MAX_CAND)
 #undef A
     int64_t maxnp = 0;
-    for (int r = 0; r < n; ++r) maxnp = std::max<int64_t>(maxnp, specs[r].n_solvent);
+    int32_t maxnv = 0;
+    for (int r = 0; r < n; ++r) {
+        maxnp = std::max<int64_t>(maxnp, specs[r].n_solvent);
+        maxnv = std::max<int32_t>(maxnv, specs[r].n_vertices);
+    }
+    c->mem_lds = (size_t)3 * maxnv * sizeof(double);
+    if (c->mem_lds > 65536 &&
+        hipFuncSetAttribute((const void *)k_membrane, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)c->mem_lds) != hipSuccess) {
+        irbc_destroy(c);
+        return fail(IRBC_ERR_ARG, "irbc_create: %d vertices need %zu B of LDS", maxnv, (size_t)3 * maxnv * 8);
+    }
     if (rc == IRBC_OK) rc = c->alloc(c->stage_f, 6 * maxnp);
     if (rc == IRBC_OK) rc = c->alloc(c->stage_i, 2 * maxnp);
     if (rc != IRBC_OK) {
@@ -451,7 +463,7 @@ int irbc_destroy(irbc_ctx *c)
     if (!c) return IRBC_OK;
     if (c->st) hipStreamSynchronize(c->st);
     for (void *p : c->allocs) hipFree(p);
-    for (int k = 0; k < 3; ++k)
+    for (int k = 0; k < 4; ++k)
         for (auto &pr : c->pend[k]) {
             hipEventDestroy(pr.first);
             hipEventDestroy(pr.second);
@@ -957,6 +969,16 @@ static int launch_llist(irbc_ctx *c, int staged)
     return IRBC_OK;
 }
 
+static int launch_stream(irbc_ctx *c)
+{
+    k_stream<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab,
+                                                         c->lloff, c->nearq);
+    k_stream_near<<<dim3(64, (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(c->d_rep, c->lloff, c->llent,
+                                                                      c->nearq, c->view());
+    LAUNCHCHK();
+    return IRBC_OK;
+}
+
 static int ready(irbc_ctx *c, bool need_solvent)
 {
     for (int r = 0; r < c->nrep; ++r) {
@@ -992,8 +1014,9 @@ int irbc_stream(irbc_ctx *c)
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     int rc = ready(c, true);
     if (rc) return rc;
-    k_stream<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab,
-                                                         c->lloff, c->llent, c->view());
+    int rc2 = launch_stream(c);
+    if (rc2) return rc2;
+    k_membrane<<<c->nrep, IRBC_MEM_WG, c->mem_lds, c->st>>>(c->d_rep, PH_RESET_NEAR, c->mem());
     LAUNCHCHK();
     return irbc_sync(c);
 }
@@ -1013,7 +1036,7 @@ int irbc_membrane_phases(irbc_ctx *c, int phases)
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     int rc = ready(c, false);
     if (rc) return rc;
-    k_membrane<<<c->nrep, IRBC_MEM_WG, 0, c->st>>>(c->d_rep, phases, c->mem());
+    k_membrane<<<c->nrep, IRBC_MEM_WG, c->mem_lds, c->st>>>(c->d_rep, phases, c->mem());
     LAUNCHCHK();
     if (phases & PH_COUNTERS)
         for (int r = 0; r < c->nrep; ++r) advance_host(c->hr[r].spec.params, c->hr[r]);
@@ -1092,9 +1115,8 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
     if (!c || nsteps < 0) return fail(IRBC_ERR_ARG, "irbc_md_steps: bad arguments");
     int rc = ready(c, true);
     if (rc) return rc;
-    const unsigned T = (unsigned)c->z.tiles;
-    const MemView V = c->view();
     const MemDev M = c->mem();
+    const MemView V = c->view();
     k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 1);
     for (int64_t s = 0; s < nsteps; ++s) {
         bool col = false, ll = false;
@@ -1115,7 +1137,8 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
             if (rc) return rc;
         }
         if (rec) { e0 = c->event(); hipEventRecord(e0, c->st); }
-        k_stream<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab, c->lloff, c->llent, V);
+        k_stream<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab,
+                                                             c->lloff, c->nearq);
         if (rec) {
             e1 = c->event();
             hipEventRecord(e1, c->st);
@@ -1123,7 +1146,16 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
             e0 = c->event();
             hipEventRecord(e0, c->st);
         }
-        k_membrane<<<c->nrep, IRBC_MEM_WG, 0, c->st>>>(c->d_rep, PH_ALL, M);
+        k_stream_near<<<dim3(64, (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(c->d_rep, c->lloff, c->llent,
+                                                                          c->nearq, V);
+        if (rec) {
+            e1 = c->event();
+            hipEventRecord(e1, c->st);
+            c->pend[3].push_back({e0, e1});
+            e0 = c->event();
+            hipEventRecord(e0, c->st);
+        }
+        k_membrane<<<c->nrep, IRBC_MEM_WG, c->mem_lds, c->st>>>(c->d_rep, PH_ALL, M);
         if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[2].push_back({e0, e1}); }
         LAUNCHCHK();
         for (int r = 0; r < c->nrep; ++r) advance_host(c->hr[r].spec.params, c->hr[r]);
@@ -1147,7 +1179,7 @@ int irbc_profile(irbc_ctx *c, int on)
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     int rc = c->flush_prof();
     if (rc) return rc;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
         c->acc_ms[k] = 0;
         c->acc_n[k] = 0;
     }
@@ -1162,7 +1194,7 @@ int irbc_profile_read(irbc_ctx *c, double *t)
     if (!c || !t) return fail(IRBC_ERR_ARG, "irbc_profile_read: bad arguments");
     int rc = c->flush_prof();
     if (rc) return rc;
-    for (int k = 0; k < 3; ++k) {
+    for (int k = 0; k < 4; ++k) {
         t[2 * k] = c->acc_ms[k];
         t[2 * k + 1] = (double)c->acc_n[k];
     }

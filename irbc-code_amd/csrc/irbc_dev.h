@@ -81,7 +81,7 @@ struct DevRep {
     int32_t status;
     int32_t bond_draws;     /* draws consumed by the last bond step */
     int32_t nbonds;         /* bonded knobs after the last bond step */
-    int32_t pad1;
+    int32_t near_count;     /* particles queued for bounce-back this step */
     /* ---- particle buffers (packed arrays, this replica at poff) ---- */
     float *xa, *ya, *za, *vxa, *vya, *vza;
     float *xb, *yb, *zb, *vxb, *vyb, *vzb;

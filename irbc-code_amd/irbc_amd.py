@@ -364,7 +364,8 @@ class Batch:
         _chk(self.L.irbc_profile(self.h, int(every)))
 
     def profile_read(self):
-        t = np.zeros(6)
+        t = np.zeros(8)
         _chk(self.L.irbc_profile_read(self.h, _ptr(t, _d)))
         return {"stream_ms": t[0], "stream_n": int(t[1]), "collision_ms": t[2],
-                "collision_n": int(t[3]), "membrane_ms": t[4], "membrane_n": int(t[5])}
+                "collision_n": int(t[3]), "membrane_ms": t[4], "membrane_n": int(t[5]),
+                "near_ms": t[6], "near_n": int(t[7])}
