@@ -69,7 +69,8 @@ struct HostRep {
 
 struct Sizes {
     int64_t slots = 0, cells = 0, cells1 = 0, llent = 0, scr = 0;
-    int32_t nv = 0, ne = 0, nt = 0, nlig = 0, nligx = 0, nligy = 0, inc = 0;
+    int32_t nv = 0, ne = 0, nt = 0, nlig = 0, nligx = 0, nligy = 0, inc = 0, part = 0;
+    int32_t maxnv = 0, maxne = 0, maxnt = 0;
     int32_t tiles = 0, ctiles = 0, scanblk = 0;
 };
 
@@ -125,6 +126,7 @@ Sizes plan(const irbc_replica_spec *specs, int n, std::vector<DevRep> *reps,
         R.inc_ent_off = z.inc;
         R.scr_off = z.scr;
         R.rep_index = r;
+        R.part_off = z.part;
         const int32_t tiles = (int32_t)((s.n_solvent + IRBC_WG - 1) / IRBC_WG);
         const int32_t ctiles = (R.ncell + IRBC_WG - 1) / IRBC_WG;
         if (tile_rep) tile_rep->insert(tile_rep->end(), tiles, r);
@@ -156,6 +158,10 @@ Sizes plan(const irbc_replica_spec *specs, int n, std::vector<DevRep> *reps,
         z.llent += R.llent_cap;
         z.scr += 8 * (int64_t)s.n_edges + 9 * (int64_t)s.n_triangles;
         z.inc += inc_bound(s);
+        z.part += (std::max(s.n_edges, s.n_triangles) + IRBC_WG - 1) / IRBC_WG;
+        z.maxnv = std::max(z.maxnv, s.n_vertices);
+        z.maxne = std::max(z.maxne, s.n_edges);
+        z.maxnt = std::max(z.maxnt, s.n_triangles);
         if (reps) reps->push_back(R);
     }
     if (rep_first_blk) rep_first_blk->push_back(z.scanblk);
@@ -170,7 +176,8 @@ int64_t bytes_of(const Sizes &z, int n)
     b += z.cells * 8 * 8;                                 /* tapes */
     b += z.cells1 * (4 + 4);                              /* start, lloff */
     b += z.llent * 4;
-    b += (int64_t)z.nv * (15 * 8 + 12 * 4 + IRBC_MAX_CAND * (4 + 8));
+    b += (int64_t)z.nv * (19 * 8 + 12 * 4 + IRBC_MAX_CAND * (4 + 8));
+    b += (int64_t)z.nt * IRBC_TRI_KIN * 8 + (int64_t)z.part * 16 + z.slots * 8;
     b += (int64_t)z.ne * (5 * 4 + 2 * 8) + (int64_t)z.nt * (3 * 4 + 2 * 8);
     b += z.scr * 24 + (int64_t)z.inc * 4 + (int64_t)(z.nv + n) * 4;
     b += (int64_t)z.nlig * 4 + (int64_t)(z.nligx + z.nligy) * 8;
@@ -182,7 +189,10 @@ int64_t bytes_of(const Sizes &z, int n)
 
 struct irbc_ctx {
     int device = 0;
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;   /* solvent chain */
+    hipStream_t st2 = nullptr;  /* membrane chain, overlapped with the solvent's streaming */
+    hipEvent_t ev_pre = nullptr, ev_mem = nullptr;
+    int par = 0;                /* parity of the current step: picks the double-buffered state */
     int nrep = 0;
     std::vector<HostRep> hr;
     std::vector<DevRep> rep;
@@ -197,7 +207,7 @@ struct irbc_ctx {
     /* solvent */
     float *xa, *ya, *za, *vxa, *vya, *vza, *xb, *yb, *zb, *vxb, *vyb, *vzb;
     uint32_t *ida, *idb;
-    int32_t *skey, *srank, *skey2, *nearq;
+    int32_t *skey, *srank, *skey2, *nearq, *deferq;
     /* cells */
     int32_t *ccount, *cstart, *vhead, *lloff, *llcount, *llent;
     double *ctab;
@@ -207,7 +217,7 @@ struct irbc_ctx {
     double *mx, *my, *mz, *mvx, *mvy, *mvz, *mfx, *mfy, *mfz, *poff, *pon;
     int32_t *mix, *miy, *mpid, *bondi, *bondl, *knob, *vnext;
     int32_t *e1, *e2, *es1, *es2, *iedge, *t1, *t2, *t3;
-    double *l0, *kappap, *tarea, *tri_area, *wca, *slot;
+    double *l0, *kappap, *tarea, *tri_area, *wca, *slot, *upos, *part, *tri_kin;
     int32_t *inc_off, *inc_ent;
     int32_t *ncand, *cand, *draw_off, *flag, *list;
     int32_t *ligi;
@@ -216,7 +226,7 @@ struct irbc_ctx {
     float *stage_f;
     int32_t *stage_i;
     bool diag_on = false;
-    size_t mem_lds = 0; /* dynamic LDS of k_membrane: unwrapped positions, 24 B per vertex */
+
     /* profiling: HIP events around every prof_every-th step's launches */
     bool prof = false;
     int prof_every = 0;
@@ -241,7 +251,8 @@ struct irbc_ctx {
         return IRBC_OK;
     }
 
-    MemDev mem() const
+    int64_t tri_stride() const { return (int64_t)IRBC_TRI_KIN * z.nt; }
+    MemDev mem(int tbuf) const
     {
         MemDev M;
         M.x = mx; M.y = my; M.z = mz; M.vx = mvx; M.vy = mvy; M.vz = mvz;
@@ -251,21 +262,36 @@ struct irbc_ctx {
         M.t1 = t1; M.t2 = t2; M.t3 = t3;
         M.l0 = l0; M.kappap = kappap; M.tarea = tarea;
         M.tri_area = tri_area; M.wca = wca; M.slot = slot;
+        M.upos = upos; M.part = part; M.tri_kin = tri_kin + (tbuf & 1) * tri_stride();
         M.inc_off = inc_off; M.inc_ent = inc_ent;
         M.ncand = ncand; M.cand = cand; M.draw_off = draw_off; M.flag = flag; M.list = list;
         M.ligi = ligi; M.ligx = ligx; M.ligy = ligy; M.bond_u = bond_u;
         return M;
     }
-    MemView view() const
+    MemView view(int tbuf) const
     {
         MemView V;
         V.x = mx; V.y = my; V.z = mz; V.vx = mvx; V.vy = mvy; V.vz = mvz;
         V.fx = mfx; V.fy = mfy; V.fz = mfz; V.t1 = t1; V.t2 = t2; V.t3 = t3;
+        V.tri_kin = tri_kin + (tbuf & 1) * tri_stride();
         return V;
+    }
+    hipError_t sync_both() const
+    {
+        hipError_t e = hipStreamSynchronize(st);
+        if (e == hipSuccess && st2) e = hipStreamSynchronize(st2);
+        return e;
+    }
+    int wait_all()
+    {
+        HIPCHK(hipStreamSynchronize(st));
+        HIPCHK(hipStreamSynchronize(st2));
+        return IRBC_OK;
     }
     int pull(int r)
     {
-        HIPCHK(hipStreamSynchronize(st));
+        int rc_ = wait_all();
+        if (rc_) return rc_;
         HIPCHK(hipMemcpy(&rep[r], d_rep + r, sizeof(DevRep), hipMemcpyDeviceToHost));
         return IRBC_OK;
     }
@@ -288,7 +314,8 @@ struct irbc_ctx {
     }
     int flush_prof()
     {
-        HIPCHK(hipStreamSynchronize(st));
+        int rc_ = wait_all();
+        if (rc_) return rc_;
         for (int k = 0; k < 4; ++k) {
             for (auto &pr : pend[k]) {
                 float ms = 0.f;
@@ -309,7 +336,7 @@ static void top_of_step(const irbc_params &p, int64_t step, int32_t colsteps_pre
                         DevRep &R)
 {
     R.step = step;
-    R.wallspeed = irbc_wallspeed(&p, step);
+    R.wall2[0] = R.wall2[1] = irbc_wallspeed(&p, step);
     R.colsteps = colsteps_pre + 1;
     R.upd = upd_pre + 1;
     R.collide = R.colsteps == p.csteps;
@@ -362,9 +389,12 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     irbc_ctx *c = new irbc_ctx();
     c->device = device;
     c->nrep = n;
-    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
-        return fail(IRBC_ERR_HIP, "hipStreamCreate failed");
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_pre, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_mem, hipEventDisableTiming) != hipSuccess) {
+        irbc_destroy(c);
+        return fail(IRBC_ERR_HIP, "stream or event creation failed");
     }
     std::vector<ScanBlk> blks;
     std::vector<int32_t> rfb, trep, ctrep;
@@ -379,6 +409,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     A(c->xa, z.slots) A(c->ya, z.slots) A(c->za, z.slots) A(c->vxa, z.slots) A(c->vya, z.slots) A(c->vza, z.slots)
     A(c->xb, z.slots) A(c->yb, z.slots) A(c->zb, z.slots) A(c->vxb, z.slots) A(c->vyb, z.slots) A(c->vzb, z.slots)
     A(c->ida, z.slots) A(c->idb, z.slots) A(c->skey, z.slots) A(c->srank, z.slots) A(c->skey2, z.slots) A(c->nearq, z.slots)
+    A(c->deferq, 2 * z.slots)
     A(c->ccount, z.cells) A(c->cstart, z.cells1) A(c->vhead, z.cells) A(c->lloff, z.cells1)
     A(c->llcount, z.cells) A(c->llent, z.llent) A(c->ctab, z.cells * 8)
     A(c->diag.vcm, 3 * z.cells) A(c->diag.ncc, z.cells) A(c->diag.rand1, z.cells)
@@ -392,6 +423,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     A(c->t1, z.nt) A(c->t2, z.nt) A(c->t3, z.nt)
     A(c->l0, z.ne) A(c->kappap, z.ne) A(c->tarea, z.nt) A(c->tri_area, z.nt) A(c->wca, 3 * z.nv)
     A(c->slot, 3 * z.scr) A(c->inc_off, z.nv + n) A(c->inc_ent, z.inc)
+    A(c->upos, 4 * (int64_t)z.nv) A(c->part, 2 * (int64_t)z.part) A(c->tri_kin, 2 * (int64_t)IRBC_TRI_KIN * z.nt)
     A(c->ncand, z.nv) A(c->cand, (int64_t)z.nv * IRBC_MAX_CAND) A(c->draw_off, z.nv)
     A(c->flag, z.nv) A(c->list, z.nv)
     A(c->ligi, z.nlig) A(c->ligx, z.nligx) A(c->ligy, z.nligy)
@@ -403,13 +435,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
         maxnp = std::max<int64_t>(maxnp, specs[r].n_solvent);
         maxnv = std::max<int32_t>(maxnv, specs[r].n_vertices);
     }
-    c->mem_lds = (size_t)3 * maxnv * sizeof(double);
-    if (c->mem_lds > 65536 &&
-        hipFuncSetAttribute((const void *)k_membrane, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)c->mem_lds) != hipSuccess) {
-        irbc_destroy(c);
-        return fail(IRBC_ERR_ARG, "irbc_create: %d vertices need %zu B of LDS", maxnv, (size_t)3 * maxnv * 8);
-    }
+    (void)maxnv;
     if (rc == IRBC_OK) rc = c->alloc(c->stage_f, 6 * maxnp);
     if (rc == IRBC_OK) rc = c->alloc(c->stage_i, 2 * maxnp);
     if (rc != IRBC_OK) {
@@ -453,7 +479,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     HIPCHK(hipMemcpyAsync(c->d_blk, blks.data(), blks.size() * sizeof(ScanBlk), hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->ligx, lx.data(), lx.size() * 8, hipMemcpyHostToDevice, c->st));
     HIPCHK(hipMemcpyAsync(c->ligy, ly.data(), ly.size() * 8, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(c->sync_both());
     *out = c;
     return IRBC_OK;
 }
@@ -462,6 +488,7 @@ int irbc_destroy(irbc_ctx *c)
 {
     if (!c) return IRBC_OK;
     if (c->st) hipStreamSynchronize(c->st);
+    if (c->st2) hipStreamSynchronize(c->st2);
     for (void *p : c->allocs) hipFree(p);
     for (int k = 0; k < 4; ++k)
         for (auto &pr : c->pend[k]) {
@@ -469,7 +496,10 @@ int irbc_destroy(irbc_ctx *c)
             hipEventDestroy(pr.second);
         }
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    if (c->ev_pre) hipEventDestroy(c->ev_pre);
+    if (c->ev_mem) hipEventDestroy(c->ev_mem);
     if (c->st) hipStreamDestroy(c->st);
+    if (c->st2) hipStreamDestroy(c->st2);
     delete c;
     return IRBC_OK;
 }
@@ -490,7 +520,8 @@ int irbc_device_bytes(const irbc_ctx *c, int64_t *bytes)
 int irbc_sync(irbc_ctx *c)
 {
     if (!c) return fail(IRBC_ERR_ARG, "irbc_sync: null context");
-    HIPCHK(hipStreamSynchronize(c->st));
+    int rw = c->wait_all();
+    if (rw) return rw;
     HIPCHK(hipMemcpy(c->rep.data(), c->d_rep, c->nrep * sizeof(DevRep), hipMemcpyDeviceToHost));
     for (int r = 0; r < c->nrep; ++r)
         if (c->rep[r].status)
@@ -638,12 +669,15 @@ int irbc_set_mesh(irbc_ctx *c, int r, const double *xyz, const int32_t *edges, c
         inc[l3].push_back(SH + 4 * e + 2);
         inc[es2[e]].push_back(SH + 4 * e + 3);
     }
-    std::vector<int32_t> ioff(nv + 1, 0), ient;
+    std::vector<int32_t> ioff(nv + 1, 0), order;
     for (int v = 0; v < nv; ++v) {
         ioff[v + 1] = ioff[v] + (int32_t)inc[v].size();
-        ient.insert(ient.end(), inc[v].begin(), inc[v].end());
+        order.insert(order.end(), inc[v].begin(), inc[v].end());
     }
-    if ((int64_t)ient.size() > inc_bound(h.spec)) return fail(IRBC_ERR_ARG, "irbc_set_mesh: incidence overflow");
+    if ((int64_t)order.size() != inc_bound(h.spec)) return fail(IRBC_ERR_ARG, "irbc_set_mesh: every edge needs two hinge partners");
+    /* element term -> position in the vertex-major order */
+    std::vector<int32_t> ient(order.size());
+    for (size_t i = 0; i < order.size(); ++i) ient[order[i]] = (int32_t)i;
     R.Sarea0 = A0;
     R.Volume0 = V0;
     R.theta0 = theta0;
@@ -694,7 +728,7 @@ int irbc_set_vertices(irbc_ctx *c, int r, const double *x, const double *y, cons
     int32_t *id[] = {c->mix, c->miy, c->mpid, c->bondi, c->bondl, c->knob};
     for (int k = 0; k < 6; ++k)
         if (is[k]) HIPCHK(hipMemcpyAsync(id[k] + R.voff, is[k], n * 4, hipMemcpyHostToDevice, c->st));
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(c->sync_both());
     return IRBC_OK;
 }
 
@@ -706,7 +740,7 @@ int irbc_get_vertices(irbc_ctx *c, int r, double *x, double *y, double *z, doubl
     CHECK_R(c, r);
     const DevRep &R = c->rep[r];
     const size_t n = c->hr[r].spec.n_vertices;
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(c->sync_both());
     double *dd[] = {x, y, z, vx, vy, vz, fx, fy, fz, poff, pon};
     const double *ds[] = {c->mx, c->my, c->mz, c->mvx, c->mvy, c->mvz, c->mfx, c->mfy, c->mfz, c->poff, c->pon};
     for (int k = 0; k < 11; ++k)
@@ -751,11 +785,11 @@ int irbc_get_solvent(irbc_ctx *c, int r, double *x, double *y, double *z, double
 {
     CHECK_R(c, r);
     const int64_t n = c->hr[r].spec.n_solvent;
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(c->sync_both());
     k_unpermute<<<(unsigned)((n + IRBC_WG - 1) / IRBC_WG), IRBC_WG, 0, c->st>>>(c->d_rep, r, c->skey2,
                                                                                c->stage_f, c->stage_i);
     LAUNCHCHK();
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(c->sync_both());
     std::vector<float> f(6 * n);
     std::vector<int32_t> iv(2 * n);
     HIPCHK(hipMemcpy(f.data(), c->stage_f, 6 * n * 4, hipMemcpyDeviceToHost));
@@ -822,7 +856,7 @@ int irbc_set_ligands(irbc_ctx *c, int r, const int32_t *ligi)
 int irbc_get_ligands(irbc_ctx *c, int r, int32_t *ligi)
 {
     CHECK_R(c, r);
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(c->sync_both());
     HIPCHK(hipMemcpy(ligi, c->ligi + c->rep[r].loff, (size_t)c->rep[r].nlig * 4, hipMemcpyDeviceToHost));
     return IRBC_OK;
 }
@@ -838,7 +872,7 @@ int irbc_get_globals(irbc_ctx *c, int r, double *g)
     g[2] = R.Sarea0;
     g[3] = R.Volume0;
     g[4] = R.theta0;
-    g[5] = R.wallspeed;
+    g[5] = R.wall2[c->par];
     g[6] = R.status;
     g[7] = R.bond_draws;
     g[8] = (double)R.step;
@@ -873,7 +907,7 @@ int irbc_set_wallspeed(irbc_ctx *c, int r, double w)
     CHECK_R(c, r);
     int rc = c->pull(r);
     if (rc) return rc;
-    c->rep[r].wallspeed = w;
+    c->rep[r].wall2[0] = c->rep[r].wall2[1] = w;
     return c->push(r);
 }
 
@@ -942,11 +976,11 @@ static int launch_collision(irbc_ctx *c, int staged, int diag)
     int rc = launch_scan(c, staged, 0, c->ccount, c->cstart);
     if (rc) return rc;
     k_scatter<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, staged, c->skey, c->srank, c->cstart, c->skey2);
-    MemDev M = c->mem();
+    MemDev M = c->mem(c->par);
     k_vbin<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, staged, M, c->vhead, c->vnext);
     k_cell<<<CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, staged, c->cstart, c->vhead, c->vnext,
                                       c->mvx, c->mvy, c->mvz, c->cell_u, c->wall_z, c->mem_z, c->ctab,
-                                      diag, c->diag);
+                                      diag, c->diag, c->par);
     k_vrot<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, staged, M, c->ctab);
     if (staged) k_rotate<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab);
     LAUNCHCHK();
@@ -958,7 +992,7 @@ static int launch_llist(irbc_ctx *c, int staged)
     int maxnt = 0;
     for (int r = 0; r < c->nrep; ++r) maxnt = std::max(maxnt, c->rep[r].nt);
     dim3 g((unsigned)((maxnt * 64 + IRBC_WG - 1) / IRBC_WG), (unsigned)c->nrep);
-    MemView V = c->view();
+    MemView V = c->view(c->par);
     k_tri_count<<<g, IRBC_WG, 0, c->st>>>(c->d_rep, staged, V, c->mix, c->miy, c->llcount);
     int rc = launch_scan(c, staged, 1, c->llcount, c->lloff);
     if (rc) return rc;
@@ -969,12 +1003,60 @@ static int launch_llist(irbc_ctx *c, int staged)
     return IRBC_OK;
 }
 
+static unsigned near_blocks(const irbc_ctx *c)
+{
+    int64_t maxnp = 0;
+    for (const DevRep &R : c->rep) maxnp = std::max<int64_t>(maxnp, R.np);
+    return (unsigned)std::max<int64_t>(16, (maxnp + 1023) / 1024);
+}
+
+static int launch_near(irbc_ctx *c)
+{
+    const double *tk = c->tri_kin + (c->par & 1) * c->tri_stride();
+    k_near_plane<<<dim3(near_blocks(c), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
+        c->d_rep, c->lloff, c->llent, c->nearq, c->deferq, tk, c->par);
+    k_near_hit<<<dim3(std::max(16u, near_blocks(c) / 8u), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
+        c->d_rep, c->lloff, c->llent, c->deferq, tk, c->par);
+    LAUNCHCHK();
+    return IRBC_OK;
+}
+
 static int launch_stream(irbc_ctx *c)
 {
     k_stream<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab,
-                                                         c->lloff, c->nearq);
-    k_stream_near<<<dim3(64, (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(c->d_rep, c->lloff, c->llent,
-                                                                      c->nearq, c->view());
+                                                         c->lloff, c->nearq, c->par);
+    LAUNCHCHK();
+    return launch_near(c);
+}
+
+static int launch_tri_kin(irbc_ctx *c)
+{
+    k_tri_kin<<<dim3((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), (unsigned)c->nrep), IRBC_WG, 0,
+                c->st>>>(c->d_rep, c->mem(c->par));
+    LAUNCHCHK();
+    return IRBC_OK;
+}
+
+/* the membrane part of one step on stream `s`: the next step's triangle
+ * kinematics go to the other parity's buffer */
+static int launch_membrane(irbc_ctx *c, int phases, hipStream_t s)
+{
+    const MemDev M = c->mem(c->par ^ 1);
+    const unsigned nr = (unsigned)c->nrep;
+    const unsigned X = IRBC_NXCD;
+    const dim3 gv(X * (unsigned)((c->z.maxnv + IRBC_WG - 1) / IRBC_WG), nr);
+    const dim3 ge(X * (unsigned)((std::max(c->z.maxne, c->z.maxnt) + IRBC_WG - 1) / IRBC_WG), nr);
+    const dim3 gt(X * (unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), nr);
+    k_mem_begin<<<gv, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
+    if (phases & (PH_VA | PH_FORCES)) {
+        k_mem_edges<<<ge, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
+        k_mem_tris<<<gt, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
+    }
+    if (phases & (PH_FORCES | PH_EXTERNAL | PH_BONDS | PH_KICK2))
+        k_mem_gather<<<gv, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
+    if (phases & PH_KICK2)
+        k_tri_kin<<<dim3((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), nr), IRBC_WG, 0, s>>>(c->d_rep, M);
+    k_mem_finish<<<nr, IRBC_MEM_WG, 0, s>>>(c->d_rep, phases, M, c->par);
     LAUNCHCHK();
     return IRBC_OK;
 }
@@ -1014,10 +1096,12 @@ int irbc_stream(irbc_ctx *c)
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     int rc = ready(c, true);
     if (rc) return rc;
-    int rc2 = launch_stream(c);
+    int rc2 = launch_tri_kin(c);
     if (rc2) return rc2;
-    k_membrane<<<c->nrep, IRBC_MEM_WG, c->mem_lds, c->st>>>(c->d_rep, PH_RESET_NEAR, c->mem());
-    LAUNCHCHK();
+    rc2 = launch_stream(c);
+    if (rc2) return rc2;
+    rc2 = launch_membrane(c, PH_RESET_NEAR, c->st);
+    if (rc2) return rc2;
     return irbc_sync(c);
 }
 
@@ -1026,7 +1110,8 @@ int irbc_classify_solvent(irbc_ctx *c)
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     int rc = ready(c, true);
     if (rc) return rc;
-    k_classify<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->view(), c->mix, c->miy);
+    k_classify<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->view(c->par), c->mix,
+                                                           c->miy);
     LAUNCHCHK();
     return irbc_sync(c);
 }
@@ -1036,10 +1121,12 @@ int irbc_membrane_phases(irbc_ctx *c, int phases)
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     int rc = ready(c, false);
     if (rc) return rc;
-    k_membrane<<<c->nrep, IRBC_MEM_WG, c->mem_lds, c->st>>>(c->d_rep, phases, c->mem());
-    LAUNCHCHK();
-    if (phases & PH_COUNTERS)
+    rc = launch_membrane(c, phases, c->st);
+    if (rc) return rc;
+    if (phases & PH_COUNTERS) {
         for (int r = 0; r < c->nrep; ++r) advance_host(c->hr[r].spec.params, c->hr[r]);
+        c->par ^= 1;
+    }
     return irbc_sync(c);
 }
 
@@ -1063,7 +1150,7 @@ int irbc_get_cells(irbc_ctx *c, int r, double *vcm, int32_t *nc, double *ncc, do
     CHECK_R(c, r);
     const DevRep &R = c->rep[r];
     const size_t n = R.ncell;
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(c->sync_both());
     if (vcm) HIPCHK(hipMemcpy(vcm, c->diag.vcm + 3 * R.coff, 3 * n * 8, hipMemcpyDeviceToHost));
     if (nc) HIPCHK(hipMemcpy(nc, c->diag.nc + R.coff, n * 4, hipMemcpyDeviceToHost));
     double *dd[] = {ncc, rand1, rand2, dvv, the};
@@ -1078,7 +1165,7 @@ int64_t irbc_get_llist(irbc_ctx *c, int r, int64_t *off, int32_t *ent, int64_t c
     CHECK_R(c, r);
     const DevRep &R = c->rep[r];
     const size_t n = R.ncell;
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(c->sync_both());
     std::vector<int32_t> o(n + 1);
     HIPCHK(hipMemcpy(o.data(), c->lloff + R.coff1, (n + 1) * 4, hipMemcpyDeviceToHost));
     const int64_t tot = o[n];
@@ -1093,7 +1180,7 @@ int64_t irbc_get_pairs(irbc_ctx *c, int r, int32_t *cnt, int32_t *lig, int64_t c
     CHECK_R(c, r);
     const DevRep &R = c->rep[r];
     const int nv = R.nv;
-    HIPCHK(hipStreamSynchronize(c->st));
+    HIPCHK(c->sync_both());
     std::vector<int32_t> nc(nv), cd((size_t)nv * IRBC_MAX_CAND);
     HIPCHK(hipMemcpy(nc.data(), c->ncand + R.voff, nv * 4, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(cd.data(), c->cand + (int64_t)R.voff * IRBC_MAX_CAND, cd.size() * 4, hipMemcpyDeviceToHost));
@@ -1115,15 +1202,23 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
     if (!c || nsteps < 0) return fail(IRBC_ERR_ARG, "irbc_md_steps: bad arguments");
     int rc = ready(c, true);
     if (rc) return rc;
-    const MemDev M = c->mem();
-    const MemView V = c->view();
     k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 1);
+    rc = launch_tri_kin(c);
+    if (rc) return rc;
+    /* Two chains per step: the solvent chain (collision, cell lists,
+     * streaming, bounce-back) on c->st and the membrane chain on c->st2.
+     * Streaming of step s only reads the triangle kinematics of step s, so
+     * the membrane of step s (which writes those of step s+1 into the other
+     * parity's buffer) runs beside it.  The collision of step s+1 waits for
+     * the membrane of step s: it moves the vertices. */
+    const int mem_phases = PH_ALL & ~PH_RESET_NEAR; /* k_stream clears the other parity's queue */
     for (int64_t s = 0; s < nsteps; ++s) {
         bool col = false, ll = false;
         for (int r = 0; r < c->nrep; ++r) {
             col |= host_collides(c->hr[r].spec.params, c->hr[r]);
             ll |= host_llist(c->hr[r].spec.params, c->hr[r]);
         }
+        if (s > 0) HIPCHK(hipStreamWaitEvent(c->st, c->ev_mem, 0));
         hipEvent_t e0 = nullptr, e1 = nullptr;
         const bool rec = c->prof && (c->prof_step++ % c->prof_every) == 0;
         if (col) {
@@ -1136,9 +1231,16 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
             rc = launch_llist(c, 0);
             if (rc) return rc;
         }
+        HIPCHK(hipEventRecord(c->ev_pre, c->st));
+        HIPCHK(hipStreamWaitEvent(c->st2, c->ev_pre, 0));
+        if (rec) { e0 = c->event(); hipEventRecord(e0, c->st2); }
+        rc = launch_membrane(c, mem_phases, c->st2);
+        if (rc) return rc;
+        if (rec) { e1 = c->event(); hipEventRecord(e1, c->st2); c->pend[2].push_back({e0, e1}); }
+        HIPCHK(hipEventRecord(c->ev_mem, c->st2));
         if (rec) { e0 = c->event(); hipEventRecord(e0, c->st); }
         k_stream<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab,
-                                                             c->lloff, c->nearq);
+                                                             c->lloff, c->nearq, c->par);
         if (rec) {
             e1 = c->event();
             hipEventRecord(e1, c->st);
@@ -1146,24 +1248,19 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
             e0 = c->event();
             hipEventRecord(e0, c->st);
         }
-        k_stream_near<<<dim3(64, (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(c->d_rep, c->lloff, c->llent,
-                                                                          c->nearq, V);
-        if (rec) {
-            e1 = c->event();
-            hipEventRecord(e1, c->st);
-            c->pend[3].push_back({e0, e1});
-            e0 = c->event();
-            hipEventRecord(e0, c->st);
-        }
-        k_membrane<<<c->nrep, IRBC_MEM_WG, c->mem_lds, c->st>>>(c->d_rep, PH_ALL, M);
-        if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[2].push_back({e0, e1}); }
+        rc = launch_near(c);
+        if (rc) return rc;
+        if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[3].push_back({e0, e1}); }
         LAUNCHCHK();
         for (int r = 0; r < c->nrep; ++r) advance_host(c->hr[r].spec.params, c->hr[r]);
+        c->par ^= 1;
         if (c->prof && c->pend[0].size() >= 2048) {
             rc = c->flush_prof();
             if (rc) return rc;
         }
     }
+    /* later work on c->st sees the last membrane chain */
+    if (nsteps > 0) HIPCHK(hipStreamWaitEvent(c->st, c->ev_mem, 0));
     return IRBC_OK;
 }
 

@@ -51,11 +51,11 @@ struct DevRep {
     int32_t loff, nlig, nligx, nligy, lxoff, lyoff;
     int64_t llent_off; /* first entry of this replica's cell->triangle entries */
     int64_t llent_cap;
-    int32_t inc_off;   /* vertex incidence CSR: offsets at voff + r, entries at inc_ent_off */
-    int32_t inc_ent_off;
-    int64_t scr_off;   /* per-element force slots (double3) */
+    int32_t inc_off;   /* vertex-major incidence offsets at voff + r */
+    int32_t inc_ent_off; /* element term -> its position in the vertex-major order */
+    int64_t scr_off;   /* element terms (double3), stored in vertex-major order */
     int32_t rep_index;
-    int32_t pad0;
+    int32_t part_off;  /* first per-block area/volume partial of this replica */
     /* ---- parameters ---- */
     double navg, mass, Theta, dtmd, cosalpha, sinalpha, walls, ramp_speed;
     int64_t ramp_steps;
@@ -71,7 +71,7 @@ struct DevRep {
     int32_t collide, llist; /* this step runs a collision / a list update */
     int32_t rotate_pending; /* collision done, solvent rotation fused into the next stream */
     int32_t cur;            /* which particle buffer holds the state (0 = a, 1 = b) */
-    double wallspeed;
+    double wall2[2];        /* top-wall speed of this and the next step, by step parity */
     double shift[3];
     double Sarea, Volume;
     double para[3], paraV[3], paraft[3], parafdt[3];
@@ -81,7 +81,10 @@ struct DevRep {
     int32_t status;
     int32_t bond_draws;     /* draws consumed by the last bond step */
     int32_t nbonds;         /* bonded knobs after the last bond step */
-    int32_t near_count;     /* particles queued for bounce-back this step */
+    int32_t near_count[2];  /* particles queued for bounce-back, by step parity */
+    int32_t defer_count[2]; /* of those, particles that cross a triangle plane */
+    int32_t wca_count;      /* vertices inside the parasite's cutoff this step */
+    int32_t draw_total;     /* bond draws needed this step */
     /* ---- particle buffers (packed arrays, this replica at poff) ---- */
     float *xa, *ya, *za, *vxa, *vya, *vza;
     float *xb, *yb, *zb, *vxb, *vyb, *vzb;
