@@ -69,11 +69,13 @@ def init_batch(b):
     b.sync()
 
 
-def cpu_baseline(b, setup, budget_s=15.0, max_steps=200):
+def cpu_baseline(b, setup, budget_s=15.0, max_steps=5000):
     """The FP64 oracle on the host cores, on the same state, for a bounded sample."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as po
     ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        ncores = min(ncores, int(os.environ["OMP_NUM_THREADS"]))
     po.set_threads(ncores)
     o = po.Oracle(po.Params.default(**setup["params"]))
     o.set_mesh(setup["V"], setup["E"], setup["T"])
@@ -97,10 +99,14 @@ def cpu_baseline(b, setup, budget_s=15.0, max_steps=200):
 
 
 def load_traffic(name):
+    """HBM bytes per launch from a committed counter summary (its
+    `bytes_per_launch:` line), or None when there is none."""
     path = os.path.join(REPO, "profiles", name)
     if os.path.exists(path):
         with open(path) as f:
-            return json.load(f).get("bytes_per_launch")
+            for line in f:
+                if line.startswith("bytes_per_launch:"):
+                    return int(line.split(":")[1])
     return None
 
 
@@ -148,7 +154,7 @@ def main():
                    "replicas": 1, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": load_traffic("k_stream_c2_traffic.json"),
+                     "traffic": load_traffic("k_stream_c2_traffic.md"),
                      "kernel": "k_stream", "kernel_ms": k_ms, "kernel_launches_timed": prof["stream_n"]},
         "phases_ms_per_launch": {
             "stream": k_ms,

@@ -12,6 +12,7 @@
  *   ligands   occupancy i32 and grid coordinate tables
  */
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cmath>
@@ -968,17 +969,23 @@ static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t 
     return IRBC_OK;
 }
 
+/* grid for a kernel that maps blocks through xcd_tile */
+static unsigned xcd_grid(unsigned n) { return IRBC_NXCD * ((n + IRBC_NXCD - 1) / IRBC_NXCD); }
+
 static int launch_collision(irbc_ctx *c, int staged, int diag)
 {
     const unsigned T = (unsigned)c->z.tiles, CT = (unsigned)c->z.ctiles;
     if (staged) k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 0);
-    k_bin<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, staged, c->skey, c->srank, c->ccount);
+    k_bin<<<xcd_grid(T), IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, staged, c->skey, c->srank,
+                                              c->ccount);
     int rc = launch_scan(c, staged, 0, c->ccount, c->cstart);
     if (rc) return rc;
-    k_scatter<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, staged, c->skey, c->srank, c->cstart, c->skey2);
+    k_scatter<<<xcd_grid(T), IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, staged, c->skey, c->srank,
+                                                  c->cstart, c->skey2);
     MemDev M = c->mem(c->par);
     k_vbin<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, staged, M, c->vhead, c->vnext);
-    k_cell<<<CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, staged, c->cstart, c->vhead, c->vnext,
+    k_cell<<<xcd_grid(CT), IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, staged, c->cstart, c->vhead,
+                                                c->vnext,
                                       c->mvx, c->mvy, c->mvz, c->cell_u, c->wall_z, c->mem_z, c->ctab,
                                       diag, c->diag, c->par);
     k_vrot<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, staged, M, c->ctab);
@@ -1238,15 +1245,20 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
         if (rc) return rc;
         if (rec) { e1 = c->event(); hipEventRecord(e1, c->st2); c->pend[2].push_back({e0, e1}); }
         HIPCHK(hipEventRecord(c->ev_mem, c->st2));
-        if (rec) { e0 = c->event(); hipEventRecord(e0, c->st); }
-        k_stream<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab,
-                                                             c->lloff, c->nearq, c->par);
         if (rec) {
+            /* events bound to the dispatch itself: the kernel's own duration */
+            e0 = c->event();
             e1 = c->event();
-            hipEventRecord(e1, c->st);
+            hipExtLaunchKernelGGL(k_stream, dim3((unsigned)c->z.tiles), dim3(IRBC_WG), 0, c->st, e0, e1, 0,
+                                  c->d_rep, (const int32_t *)c->d_tile_rep,
+                                  (const int32_t *)c->skey2, (const double *)c->ctab,
+                                  (const int32_t *)c->lloff, c->nearq, c->par);
             c->pend[0].push_back({e0, e1});
             e0 = c->event();
             hipEventRecord(e0, c->st);
+        } else {
+            k_stream<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab,
+                                                                 c->lloff, c->nearq, c->par);
         }
         rc = launch_near(c);
         if (rc) return rc;

@@ -165,6 +165,18 @@ __device__ __forceinline__ int32_t cell_of(int fx, int fy, int fz, int Lx, int L
     return 0;
 }
 
+/* Workgroups are dealt to the 8 XCDs round-robin by launch index.  A grid
+ * of xcd_grid(n) blocks mapped through xcd_tile hands each XCD one
+ * contiguous eighth of the n logical tiles, so writes that land in the same
+ * cache lines (neighbouring cells of a sort) come from one XCD's L2.
+ * Blocks with xcd_tile(n) >= n do nothing. */
+#define IRBC_NXCD 8
+__device__ __forceinline__ int xcd_tile(int n)
+{
+    const int per = (n + IRBC_NXCD - 1) / IRBC_NXCD;
+    return (int)(blockIdx.x % IRBC_NXCD) * per + (int)(blockIdx.x / IRBC_NXCD);
+}
+
 __device__ __forceinline__ void cross3(const double a[3], const double b[3], double c[3])
 {
     c[0] = (a[1] * b[2]) - (b[1] * a[2]);
