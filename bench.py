@@ -1,8 +1,8 @@
 """Benchmark of the iRBC adhesive-dynamics step on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--config c2|c3|c3plain] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--no-cpu] [--no-batched]
 
-Default workload (one GPU): a single replica at the larger size — box
+Headline workload (one GPU): a single replica at the larger size — box
 64x64x48 cells at density 10 (1,966,080 MPCD particles, FP32 state with FP64
 arithmetic and exact cell sums), a 642-vertex icosphere of radius 8, 500
 knobs, kon = 0.005, koff = 0.015, shear rate 0.05 (top wall 2.4).  A step is
@@ -11,9 +11,10 @@ step, an SRD collision every 10th, membrane forces, integrator and bond
 kinetics every step, cell->triangle lists every 104th.
 
 Prints ONE JSON line with the throughput, the HBM roofline of the dominant
-kernel (the streaming kernel, timed with HIP events on its own stream) and
+kernel (the streaming kernel, timed with HIP events bound to its dispatch),
 the CPU oracle's rate on the host cores for a bounded sample of the same
-workload.
+workload, and the batched ensemble (64 replicas of 32x32x24) against the
+same replica run alone, in replica-steps/s.
 """
 import argparse
 import json
@@ -21,55 +22,17 @@ import os
 import sys
 import time
 
-import numpy as np
-
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "irbc-code_amd"))
 
+import ensemble as en  # noqa: E402
 import irbc_amd as ia  # noqa: E402
-import scenario as sc  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 STREAM_BYTES_PER_PARTICLE = 48.0  # x,y,z,vx,vy,vz FP32 read + write
 
 
-def c2_params():
-    return dict(Lx=64, Ly=64, Lz=48, walls=0.05 * 48, ramp_steps=0, onrate=0.005, offrate=0.015)
-
-
-def build_replica(b, r, kw, nv, radius, nknobs, seed, density=10):
-    p = ia.Params.default(**kw)
-    V, E, T = sc.mesh_for(nv)
-    V = sc.place_reference(V * radius, p.Lx, p.Ly)
-    ns = int(density * p.Lx * p.Ly * p.Lz)
-    sol = sc.solvent(ns, p.Lx, p.Ly, p.Lz, seed)
-    vel = sc.vertex_velocities(nv, seed + 1, pmass=p.pmass)
-    knob = sc.knobs_uniform(nv, nknobs, seed + 2)
-    pd, pi = sc.parasite_state(V, seed + 3)
-    b.set_mesh(r, V, E, T)
-    b.set_solvent(r, *sol)
-    b.set_vertices(r, vx=vel[:, 0], vy=vel[:, 1], vz=vel[:, 2], knob=knob)
-    b.set_parasite(r, pd, pi)
-    b.set_counters(r, 0, -1, 0)
-    return dict(V=V, E=E, T=T, sol=sol, vel=vel, knob=knob, para=(pd, pi), params=kw)
-
-
-def spec_for(kw, nv, seed, density=10):
-    p = ia.Params.default(**kw)
-    ns = int(density * p.Lx * p.Ly * p.Lz)
-    V, E, T = sc.mesh_for(nv)
-    return ia.make_spec(p, ns, nv, len(E), len(T), seed=seed)
-
-
-def init_batch(b):
-    """maincode.cpp:383-391: forces at rest, inside/outside flags, cell lists."""
-    b.membrane_phases(ia.PH_VA | ia.PH_FORCES)
-    b.classify()
-    b.llist_update()
-    b.sync()
-
-
-def cpu_baseline(b, setup, budget_s=15.0, max_steps=5000):
+def cpu_baseline(b, d, st, budget_s=15.0, max_steps=5000):
     """The FP64 oracle on the host cores, on the same state, for a bounded sample."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as po
@@ -77,8 +40,11 @@ def cpu_baseline(b, setup, budget_s=15.0, max_steps=5000):
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():
         ncores = min(ncores, int(os.environ["OMP_NUM_THREADS"]))
     po.set_threads(ncores)
-    o = po.Oracle(po.Params.default(**setup["params"]))
-    o.set_mesh(setup["V"], setup["E"], setup["T"])
+    p, pp = en.params(d), po.Params()
+    for f, _ in ia.Params._fields_:
+        setattr(pp, f, getattr(p, f))
+    o = po.Oracle(pp)
+    o.set_mesh(st["V"], st["E"], st["T"])
     s = b.solvent(0)
     o.set_solvent(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"], s["indio"])
     v = b.vertices(0)
@@ -95,7 +61,7 @@ def cpu_baseline(b, setup, budget_s=15.0, max_steps=5000):
     ns = len(s["x"])
     return {"value": ns * n / dt, "unit": "particle-steps/s", "cores": int(ncores), "kind": "port",
             "sample": "FP64 oracle, same 64x64x48 state, %d MD steps (%d collisions), %.1f s"
-                      % (n, (n + 1) // 10, dt)}
+                      % (n, n // 10, dt)}
 
 
 def load_traffic(name):
@@ -110,13 +76,44 @@ def load_traffic(name):
     return None
 
 
+def timed(b, steps, warmup):
+    b.md_steps(warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.md_steps(steps)
+    b.sync()
+    return time.perf_counter() - t0
+
+
+def batched(steps, warmup):
+    """64 replicas of 32x32x24 in one batch vs the first of them alone."""
+    ds = en.uniform()
+    b = en.batch(ds)
+    dt_b = timed(b, steps, warmup)
+    np_b = sum(en.n_solvent(d) for d in ds)
+    b.close()
+    a = en.batch(ds[:1])
+    dt_a = timed(a, steps, warmup)
+    a.close()
+    rb = len(ds) * steps / dt_b
+    ra = steps / dt_a
+    return {"workload": "64 replicas of 32x32x24 cells, density 10 (245760 particles each), "
+                        "642-vertex icosphere r=6, 200-800 knobs, kon/koff log-uniform [1e-3,1e-1]",
+            "steps": steps, "warmup": warmup,
+            "batched_replica_steps_per_s": rb, "plain_replica_steps_per_s": ra,
+            "speedup": rb / ra, "batched_ms_per_step": dt_b * 1e3 / steps,
+            "plain_ms_per_step": dt_a * 1e3 / steps,
+            "batched_particle_steps_per_s": np_b * steps / dt_b}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--config", default="c2")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=2000)
     ap.add_argument("--profile-every", type=int, default=4)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -125,12 +122,11 @@ def main():
               % a.gpus, file=sys.stderr)
         sys.exit(2)
     ia.load()
-    kw = c2_params()
-    spec = spec_for(kw, 642, 12345)
-    b = ia.Batch([spec])
-    setup = build_replica(b, 0, kw, 642, 8.0, 500, 12345)
-    init_batch(b)
-    ns = spec.n_solvent
+    d = en.single()
+    b = ia.Batch([en.spec(d)])
+    st = en.load(b, 0, d)
+    en.start(b)
+    ns = en.n_solvent(d)
     b.md_steps(a.warmup)
     b.sync()
     b.profile(a.profile_every)
@@ -163,7 +159,10 @@ def main():
             "stream_near": prof["near_ms"] / max(prof["near_n"], 1)},
     }
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(b, setup)
+        line["cpu_baseline"] = cpu_baseline(b, d, st)
+    b.close()
+    if not a.no_batched:
+        line["batched"] = batched(a.batched_steps, 100)
     print(json.dumps(line))
 
 

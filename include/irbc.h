@@ -151,6 +151,19 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
  * irbc_md_steps; costs ~56 B per cell per collision */
 int irbc_set_diagnostics(irbc_ctx *ctx, int on);
 
+/* ---- time series and asynchronous use (request loops) ---- */
+/* every > 0: after every `every`-th step of irbc_md_steps record
+ * IRBC_SERIES_REC = 8 doubles (step, membrane centroid x y z with periodic
+ * images unwrapped, centroid velocity vx vy vz, bonded knobs) into a device
+ * buffer of `cap` records; every = 0: off.  Resets the count.  No
+ * counterpart in the reference, which writes frames instead
+ * (maincode.cpp:746-754). */
+int irbc_set_series(irbc_ctx *ctx, int r, int32_t every, int32_t cap);
+/* copies min(count, cap) records; returns the count or < 0 */
+int64_t irbc_get_series(irbc_ctx *ctx, int r, double *out, int64_t cap);
+/* 1: all launched work of the context has finished; 0: still running */
+int irbc_query(irbc_ctx *ctx);
+
 /* ---- measurement: HIP-event timing on the context's stream ---- */
 /* every > 0: bracket the launches of every every-th step of irbc_md_steps
  * with events; 0: off.  Resets the accumulators. */

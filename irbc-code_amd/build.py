@@ -4,8 +4,8 @@
 
 Outputs:
   irbc-code_amd/libirbc_amd.so   HIP kernels + C ABI (hipcc, gfx950)
+  irbc-code_amd/libirbc_host.so  scenario set-up, reference-named functions
   irbc-code_amd/irbc_main        reference-style command-line driver
-  irbc-code_amd/librefnames.so   reference-named functions over the C ABI
   irbc-code_amd/copy_probe       HBM copy-bandwidth probe (counter calibration)
   oracle/liboracle.so            CPU checker (test infrastructure)
   oracle/_ref/ref_run            harnessed reference build (only where the
@@ -47,17 +47,17 @@ def build_probe():
 
 
 def build_host():
-    inc = os.path.join(REPO, "include")
+    """Host library (scenario set-up, reference-named functions, C entry
+    points) and the command-line driver, both over libirbc_amd.so."""
+    inc = ["-I" + os.path.join(REPO, "include"), "-I" + os.path.join(HERE, "host")]
     lib = ["-L" + HERE, "-lirbc_amd", "-Wl,-rpath,$ORIGIN"]
-    src_main = os.path.join(HERE, "host", "irbc_main.cpp")
-    src_host = os.path.join(HERE, "host", "scenario.cpp")
-    if os.path.exists(src_main):
-        run(["g++", "-O2", "-std=c++17", "-I" + inc, "-o", os.path.join(HERE, "irbc_main"),
-             src_main, src_host] + lib)
-    src_ref = os.path.join(HERE, "host", "refnames.cpp")
-    if os.path.exists(src_ref):
-        run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I" + inc, "-o",
-             os.path.join(HERE, "librefnames.so"), src_ref, src_host] + lib)
+    src = [os.path.join(HERE, "host", f) for f in ("scenario.cpp", "refnames.cpp", "host_capi.cpp")]
+    out = os.path.join(HERE, "libirbc_host.so")
+    run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall"] + inc + ["-o", out + ".tmp"] + src + lib)
+    os.replace(out + ".tmp", out)
+    run(["g++", "-O2", "-std=c++17", "-Wall"] + inc + ["-o", os.path.join(HERE, "irbc_main"),
+         os.path.join(HERE, "host", "irbc_main.cpp"), "-L" + HERE, "-lirbc_host", "-lirbc_amd",
+         "-Wl,-rpath,$ORIGIN"])
 
 
 def build_oracle():

@@ -922,6 +922,51 @@ int irbc_set_seed(irbc_ctx *c, int r, uint64_t seed)
     return c->push(r);
 }
 
+int irbc_set_series(irbc_ctx *c, int r, int32_t every, int32_t cap)
+{
+    CHECK_R(c, r);
+    if (every < 0 || cap < 0 || (every > 0 && cap == 0))
+        return fail(IRBC_ERR_ARG, "irbc_set_series: every >= 0 and cap >= 1 when on");
+    int rc = c->pull(r);
+    if (rc) return rc;
+    DevRep &R = c->rep[r];
+    if (every > 0 && cap > R.series_cap) {
+        double *buf = nullptr;
+        rc = c->alloc(buf, (int64_t)cap * IRBC_SERIES_REC);
+        if (rc) return rc;
+        R.series = buf; /* freed with the context */
+    }
+    R.series_every = every;
+    if (every > 0) R.series_cap = std::max(cap, R.series_cap);
+    R.series_n = 0;
+    return c->push(r);
+}
+
+int64_t irbc_get_series(irbc_ctx *c, int r, double *out, int64_t cap)
+{
+    CHECK_R(c, r);
+    int rc = c->pull(r);
+    if (rc) return rc;
+    const DevRep &R = c->rep[r];
+    const int64_t n = R.series_n;
+    if (out && n > 0 && R.series)
+        HIPCHK(hipMemcpy(out, R.series, std::min(n, cap) * IRBC_SERIES_REC * sizeof(double),
+                         hipMemcpyDeviceToHost));
+    return n;
+}
+
+int irbc_query(irbc_ctx *c)
+{
+    if (!c) return fail(IRBC_ERR_ARG, "irbc_query: null context");
+    hipStream_t ss[2] = {c->st, c->st2};
+    for (hipStream_t q : ss) {
+        const hipError_t e = hipStreamQuery(q);
+        if (e == hipErrorNotReady) return 0;
+        if (e != hipSuccess) return fail(IRBC_ERR_HIP, "irbc_query: %s", hipGetErrorString(e));
+    }
+    return 1;
+}
+
 int irbc_use_tapes(irbc_ctx *c, int r, int on)
 {
     CHECK_R(c, r);

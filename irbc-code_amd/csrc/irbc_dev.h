@@ -38,6 +38,8 @@
 #define IRBC_RS_MEM 4u
 #define IRBC_RS_BOND 5u
 
+#define IRBC_SERIES_REC 8      /* doubles per time-series record */
+
 struct DevRep {
     /* ---- geometry and packing ---- */
     int32_t Lx, Ly, Lz, ncell;
@@ -85,6 +87,10 @@ struct DevRep {
     int32_t defer_count[2]; /* of those, particles that cross a triangle plane */
     int32_t wca_count;      /* vertices inside the parasite's cutoff this step */
     int32_t draw_total;     /* bond draws needed this step */
+    /* time series: every `series_every` steps one record of IRBC_SERIES_REC
+     * doubles (step, centroid x y z unwrapped, centroid velocity, bonds) */
+    int32_t series_every, series_cap, series_n, series_pad;
+    double *series;
     /* ---- particle buffers (packed arrays, this replica at poff) ---- */
     float *xa, *ya, *za, *vxa, *vya, *vza;
     float *xb, *yb, *zb, *vxb, *vyb, *vzb;

@@ -25,8 +25,10 @@ EXPORTS = (
     "irbc_stream", "irbc_classify_solvent", "irbc_membrane_phases", "irbc_membrane_va",
     "irbc_membrane_forces", "irbc_bond_step", "irbc_get_cells", "irbc_get_llist",
     "irbc_get_pairs", "irbc_md_steps", "irbc_set_diagnostics", "irbc_profile",
-    "irbc_profile_read",
+    "irbc_profile_read", "irbc_set_series", "irbc_get_series", "irbc_query",
 )
+
+SERIES_FIELDS = ("step", "x", "y", "z", "vx", "vy", "vz", "bonds")
 
 _d = C.POINTER(C.c_double)
 _i = C.POINTER(C.c_int32)
@@ -124,6 +126,10 @@ def load():
     L.irbc_set_solvent.argtypes = [C.c_void_p, C.c_int, C.c_int64] + [_d] * 6 + [_i]
     L.irbc_md_steps.argtypes = [C.c_void_p, C.c_int64]
     L.irbc_membrane_va.argtypes = [C.c_void_p, C.c_int, _d, _d]
+    L.irbc_set_series.argtypes = [C.c_void_p, C.c_int, C.c_int32, C.c_int32]
+    L.irbc_get_series.restype = C.c_int64
+    L.irbc_get_series.argtypes = [C.c_void_p, C.c_int, _d, C.c_int64]
+    L.irbc_query.argtypes = [C.c_void_p]
     _lib = L
     return L
 
@@ -355,6 +361,20 @@ class Batch:
 
     def md_steps(self, n):
         _chk(self.L.irbc_md_steps(self.h, int(n)))
+
+    def set_series(self, r, every, cap):
+        """Record (step, centroid xyz, centroid velocity, bonds) every `every` steps."""
+        _chk(self.L.irbc_set_series(self.h, r, int(every), int(cap)))
+
+    def series(self, r=0):
+        n = _chk(self.L.irbc_get_series(self.h, r, None, 0))
+        out = np.zeros((max(n, 1), len(SERIES_FIELDS)))
+        _chk(self.L.irbc_get_series(self.h, r, _ptr(out, _d), n))
+        return out[:n]
+
+    def done(self):
+        """True when all launched work has finished (non-blocking)."""
+        return _chk(self.L.irbc_query(self.h)) == 1
 
     def set_diagnostics(self, on=True):
         _chk(self.L.irbc_set_diagnostics(self.h, 1 if on else 0))

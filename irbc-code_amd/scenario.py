@@ -110,12 +110,17 @@ def vertex_velocities(nv, seed, kT=1.0, pmass=10.0):
     return -vmax + 2.0 * vmax * rng.random((nv, 3))
 
 
-def parasite_state(V_placed, seed, kT=1.0, PRmass=10.0):
-    """Bead 2 units behind the cell centre along -x (maincode.cpp:212-225)."""
+def parasite_state(V_placed, seed, kT=1.0, PRmass=10.0, cutoff=1.7):
+    """Bead behind the cell centre along -x (maincode.cpp:212-225): 2 units
+    as in the reference, less in a small cell, so that the bead starts at
+    least a WCA cutoff (plus a margin) from every vertex; a bead that starts
+    on the membrane meets forces of order 1e20."""
     rng = np.random.default_rng(seed)
     c = np.array([V_placed[:, 0].mean(), V_placed[:, 1].mean(), V_placed[:, 2].mean()])
+    r_in = np.sqrt(((V_placed - c) ** 2).sum(1)).min()
+    off = min(2.0, max(0.0, r_in - (cutoff + 0.1)))
     vmax = np.sqrt(3.0 * kT / PRmass)
     d = np.zeros(12)
-    d[0:3] = c - np.array([2.0, 0.0, 0.0])
+    d[0:3] = c - np.array([off, 0.0, 0.0])
     d[3:6] = -vmax + 2.0 * vmax * rng.random(3)
     return d, np.zeros(3, np.int32)
