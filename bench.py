@@ -27,6 +27,7 @@ sys.path.insert(0, os.path.join(REPO, "irbc-code_amd"))
 
 import ensemble as en  # noqa: E402
 import irbc_amd as ia  # noqa: E402
+import serve  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 STREAM_BYTES_PER_PARTICLE = 48.0  # x,y,z,vx,vy,vz FP32 read + write
@@ -106,6 +107,32 @@ def batched(steps, warmup):
             "batched_particle_steps_per_s": np_b * steps / dt_b}
 
 
+def served(n, rate, steps, budget_gb, plain_n=3):
+    """Poisson arrivals of uneven-ensemble requests through the request
+    loop, against the same requests run one at a time."""
+    reqs = serve.poisson_requests(n, rate, steps, seed=1)
+    srv = serve.Server(int(budget_gb * 2 ** 30), chunk=250)
+    srv.run(reqs)
+    out = serve.summary(reqs, srv.makespan)
+    t_plain, s_plain = 0.0, 0
+    for r in reqs[:plain_n]:
+        a = en.batch([r.replica])
+        t0 = time.perf_counter()
+        a.md_steps(r.steps)
+        a.sync()
+        t_plain += time.perf_counter() - t0
+        s_plain += r.steps
+        a.close()
+    out.update({"workload": "%d requests, Poisson arrivals at %.1f/s, %d steps each, boxes 24^3..64x64x48 "
+                            "at density 5-10, 162/642/2562-vertex ellipsoids" % (n, rate, steps),
+                "hbm_budget_GiB": budget_gb, "chunk_steps": srv.chunk,
+                "micro_batches": srv.stats["groups"], "max_concurrent_micro_batches": srv.stats["max_active_groups"],
+                "plain_replica_steps_per_s": s_plain / t_plain,
+                "plain_sample": "first %d requests run alone, one after another" % plain_n})
+    out["speedup_vs_plain"] = out["replica_steps_per_s"] / out["plain_replica_steps_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +141,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=2000)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--served-requests", type=int, default=24)
+    ap.add_argument("--served-rate", type=float, default=4.0)
+    ap.add_argument("--served-steps", type=int, default=5000)
     ap.add_argument("--profile-every", type=int, default=4)
     a = ap.parse_args()
     if a.gpus != 1:
@@ -163,6 +194,8 @@ def main():
     b.close()
     if not a.no_batched:
         line["batched"] = batched(a.batched_steps, 100)
+    if not a.no_served:
+        line["served"] = served(a.served_requests, a.served_rate, a.served_steps, 16.0)
     print(json.dumps(line))
 
 

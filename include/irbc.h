@@ -147,6 +147,12 @@ int64_t irbc_get_pairs(irbc_ctx *ctx, int r, int32_t *cnt, int32_t *lig, int64_t
 
 /* ---- the time loop ---- */
 int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
+/* launch-shape options (no counterpart in the reference):
+ * IRBC_OPT_MEMBRANE_FUSED  1 (default): the membrane part of a step is one
+ *   workgroup per replica; 0: five grid-wide kernels.  Both give the same
+ *   forces up to the summation order of the global area and volume. */
+#define IRBC_OPT_MEMBRANE_FUSED 1
+int irbc_set_option(irbc_ctx *ctx, int option, int value);
 /* keep per-cell results (irbc_get_cells) also for collisions run inside
  * irbc_md_steps; costs ~56 B per cell per collision */
 int irbc_set_diagnostics(irbc_ctx *ctx, int on);

@@ -194,6 +194,7 @@ struct irbc_ctx {
     hipStream_t st2 = nullptr;  /* membrane chain, overlapped with the solvent's streaming */
     hipEvent_t ev_pre = nullptr, ev_mem = nullptr;
     int par = 0;                /* parity of the current step: picks the double-buffered state */
+    int mem_fused = 1;          /* membrane step as one workgroup per replica (1) or five grid kernels (0) */
     int nrep = 0;
     std::vector<HostRep> hr;
     std::vector<DevRep> rep;
@@ -1095,6 +1096,11 @@ static int launch_membrane(irbc_ctx *c, int phases, hipStream_t s)
 {
     const MemDev M = c->mem(c->par ^ 1);
     const unsigned nr = (unsigned)c->nrep;
+    if (c->mem_fused) {
+        k_mem_step<<<nr, IRBC_MEM_WG, 0, s>>>(c->d_rep, phases, M, c->par);
+        LAUNCHCHK();
+        return IRBC_OK;
+    }
     const unsigned X = IRBC_NXCD;
     const dim3 gv(X * (unsigned)((c->z.maxnv + IRBC_WG - 1) / IRBC_WG), nr);
     const dim3 ge(X * (unsigned)((std::max(c->z.maxne, c->z.maxnt) + IRBC_WG - 1) / IRBC_WG), nr);
@@ -1319,6 +1325,18 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
     /* later work on c->st sees the last membrane chain */
     if (nsteps > 0) HIPCHK(hipStreamWaitEvent(c->st, c->ev_mem, 0));
     return IRBC_OK;
+}
+
+int irbc_set_option(irbc_ctx *c, int option, int value)
+{
+    if (!c) return fail(IRBC_ERR_ARG, "null context");
+    switch (option) {
+    case IRBC_OPT_MEMBRANE_FUSED:
+        c->mem_fused = value != 0;
+        return IRBC_OK;
+    default:
+        return fail(IRBC_ERR_ARG, "irbc_set_option: unknown option %d", option);
+    }
 }
 
 int irbc_set_diagnostics(irbc_ctx *c, int on)

@@ -25,8 +25,10 @@ EXPORTS = (
     "irbc_stream", "irbc_classify_solvent", "irbc_membrane_phases", "irbc_membrane_va",
     "irbc_membrane_forces", "irbc_bond_step", "irbc_get_cells", "irbc_get_llist",
     "irbc_get_pairs", "irbc_md_steps", "irbc_set_diagnostics", "irbc_profile",
-    "irbc_profile_read", "irbc_set_series", "irbc_get_series", "irbc_query",
+    "irbc_profile_read", "irbc_set_series", "irbc_get_series", "irbc_query", "irbc_set_option",
 )
+
+OPT_MEMBRANE_FUSED = 1
 
 SERIES_FIELDS = ("step", "x", "y", "z", "vx", "vy", "vz", "bonds")
 
@@ -375,6 +377,9 @@ class Batch:
     def done(self):
         """True when all launched work has finished (non-blocking)."""
         return _chk(self.L.irbc_query(self.h)) == 1
+
+    def set_option(self, option, value):
+        _chk(self.L.irbc_set_option(self.h, int(option), int(value)))
 
     def set_diagnostics(self, on=True):
         _chk(self.L.irbc_set_diagnostics(self.h, 1 if on else 0))
