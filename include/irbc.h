@@ -152,10 +152,27 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
  *   workgroup per replica; 0: five grid-wide kernels.  Both give the same
  *   forces up to the summation order of the global area and volume. */
 #define IRBC_OPT_MEMBRANE_FUSED 1
+/* IRBC_OPT_GRAPHS  1: irbc_md_steps captures chunks of 10 steps as HIP
+ *   graphs (one per chunk shape) and replays them; 0 (default): launches
+ *   kernel by kernel.  Ignored while event profiling is on. */
+#define IRBC_OPT_GRAPHS 2
 int irbc_set_option(irbc_ctx *ctx, int option, int value);
 /* keep per-cell results (irbc_get_cells) also for collisions run inside
  * irbc_md_steps; costs ~56 B per cell per collision */
 int irbc_set_diagnostics(irbc_ctx *ctx, int on);
+
+/* ---- checkpoint / resume (no counterpart in the reference) ---- */
+/* A replica's resumable state as one flat byte buffer: solvent in its
+ * current order, membrane, bonds, ligand occupancy, parasite, step
+ * counters, stream key and the cell->triangle lists of the last update.
+ * Loading it into a replica of the same sizes (any context, any slot) and
+ * continuing gives the same bits as never stopping.  Save between calls of
+ * irbc_md_steps. */
+int64_t irbc_checkpoint_bytes(irbc_ctx *ctx, int r);
+/* returns the bytes written or < 0 */
+int64_t irbc_checkpoint_save(irbc_ctx *ctx, int r, void *buf, int64_t cap);
+/* the replica's mesh must be loaded (irbc_set_mesh) first */
+int irbc_checkpoint_load(irbc_ctx *ctx, int r, const void *buf, int64_t n);
 
 /* ---- time series and asynchronous use (request loops) ---- */
 /* every > 0: after every `every`-th step of irbc_md_steps record

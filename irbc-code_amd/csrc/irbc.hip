@@ -21,6 +21,7 @@
 #include <cstring>
 #include <iterator>
 #include <string>
+#include <map>
 #include <vector>
 
 #include "../../include/irbc.h"
@@ -195,6 +196,8 @@ struct irbc_ctx {
     hipEvent_t ev_pre = nullptr, ev_mem = nullptr;
     int par = 0;                /* parity of the current step: picks the double-buffered state */
     int mem_fused = 1;          /* membrane step as one workgroup per replica (1) or five grid kernels (0) */
+    int use_graphs = 0;         /* irbc_md_steps replays captured chunks of IRBC_GRAPH_STEPS steps */
+    std::map<uint64_t, hipGraphExec_t> graphs;
     int nrep = 0;
     std::vector<HostRep> hr;
     std::vector<DevRep> rep;
@@ -498,6 +501,7 @@ int irbc_destroy(irbc_ctx *c)
             hipEventDestroy(pr.second);
         }
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
+    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
     if (c->ev_pre) hipEventDestroy(c->ev_pre);
     if (c->ev_mem) hipEventDestroy(c->ev_mem);
     if (c->st) hipStreamDestroy(c->st);
@@ -1015,6 +1019,8 @@ static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t 
     return IRBC_OK;
 }
 
+#define IRBC_GRAPH_STEPS 10 /* steps per captured chunk: one collision period */
+
 /* grid for a kernel that maps blocks through xcd_tile */
 static unsigned xcd_grid(unsigned n) { return IRBC_NXCD * ((n + IRBC_NXCD - 1) / IRBC_NXCD); }
 
@@ -1253,77 +1259,353 @@ int64_t irbc_get_pairs(irbc_ctx *c, int r, int32_t *cnt, int32_t *lig, int64_t c
     return t;
 }
 
+/* ---------------- checkpoint / resume ---------------- */
+
+/* A replica's resumable state: what the next step reads and cannot derive.
+ * Triangle kinematics, near queues and the grid shift are recomputed at the
+ * start of irbc_md_steps; the cell->triangle lists are saved because they
+ * belong to the last list update, not to the current positions. */
+struct CkptHeader {
+    char magic[8];
+    int32_t version, tapes;
+    irbc_params params;
+    int64_t ns, ncell, llen;
+    int32_t nv, ne, nt, nlig;
+    int64_t step;
+    int32_t colsteps, upd;
+    uint64_t seed;
+    double wallspeed;
+};
+
+static int64_t ckpt_bytes(const irbc_ctx *c, int r, int64_t llen)
+{
+    const HostRep &h = c->hr[r];
+    const int64_t ns = h.spec.n_solvent, nv = h.spec.n_vertices;
+    const int64_t ncell = c->rep[r].ncell, nlig = c->rep[r].nlig;
+    return (int64_t)sizeof(CkptHeader) + ns * (6 * 4 + 4) + nv * (9 * 8 + 6 * 4) + nlig * 4 + 12 * 8 + 4 * 4 +
+           (ncell + 1) * 4 + llen * 4;
+}
+
+static int64_t llist_len(irbc_ctx *c, int r)
+{
+    const DevRep &R = c->rep[r];
+    int32_t a = 0, b = 0;
+    if (hipMemcpy(&a, c->lloff + R.coff1, 4, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(&b, c->lloff + R.coff1 + R.ncell, 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(IRBC_ERR_HIP, "checkpoint: reading the cell lists failed");
+    return (int64_t)b - a;
+}
+
+int64_t irbc_checkpoint_bytes(irbc_ctx *c, int r)
+{
+    CHECK_R(c, r);
+    HIPCHK(c->sync_both());
+    const int64_t llen = llist_len(c, r);
+    if (llen < 0) return llen;
+    return ckpt_bytes(c, r, llen);
+}
+
+int64_t irbc_checkpoint_save(irbc_ctx *c, int r, void *buf, int64_t cap)
+{
+    CHECK_R(c, r);
+    if (!buf) return fail(IRBC_ERR_ARG, "irbc_checkpoint_save: null buffer");
+    int rc = c->pull(r);
+    if (rc) return rc;
+    const DevRep &R = c->rep[r];
+    if (R.rotate_pending) return fail(IRBC_ERR_STATE, "irbc_checkpoint_save: a collision's rotation is pending");
+    const int64_t llen = llist_len(c, r);
+    if (llen < 0) return llen;
+    const int64_t need = ckpt_bytes(c, r, llen);
+    if (cap < need) return fail(IRBC_ERR_ARG, "irbc_checkpoint_save: needs %lld bytes", (long long)need);
+    const HostRep &h = c->hr[r];
+    CkptHeader H;
+    std::memset(&H, 0, sizeof H);
+    std::memcpy(H.magic, "IRBCCKP1", 8);
+    H.version = 1;
+    H.tapes = R.tapes;
+    H.params = h.spec.params;
+    H.ns = h.spec.n_solvent;
+    H.ncell = R.ncell;
+    H.llen = llen;
+    H.nv = R.nv;
+    H.ne = R.ne;
+    H.nt = R.nt;
+    H.nlig = R.nlig;
+    H.step = h.step;
+    H.colsteps = h.colsteps;
+    H.upd = h.upd;
+    H.seed = R.seed;
+    H.wallspeed = R.wall2[c->par & 1];
+    char *p = (char *)buf;
+    std::memcpy(p, &H, sizeof H);
+    p += sizeof H;
+    const bool cur = R.cur != 0;
+    const float *sf[] = {cur ? c->xb : c->xa, cur ? c->yb : c->ya, cur ? c->zb : c->za,
+                         cur ? c->vxb : c->vxa, cur ? c->vyb : c->vya, cur ? c->vzb : c->vza};
+    for (int k = 0; k < 6; ++k) {
+        HIPCHK(hipMemcpy(p, sf[k] + R.poff, H.ns * 4, hipMemcpyDeviceToHost));
+        p += H.ns * 4;
+    }
+    HIPCHK(hipMemcpy(p, (cur ? c->idb : c->ida) + R.poff, H.ns * 4, hipMemcpyDeviceToHost));
+    p += H.ns * 4;
+    const double *vd[] = {c->mx, c->my, c->mz, c->mvx, c->mvy, c->mvz, c->mfx, c->mfy, c->mfz};
+    for (int k = 0; k < 9; ++k) {
+        HIPCHK(hipMemcpy(p, vd[k] + R.voff, (size_t)H.nv * 8, hipMemcpyDeviceToHost));
+        p += (size_t)H.nv * 8;
+    }
+    const int32_t *vi[] = {c->mix, c->miy, c->mpid, c->bondi, c->bondl, c->knob};
+    for (int k = 0; k < 6; ++k) {
+        HIPCHK(hipMemcpy(p, vi[k] + R.voff, (size_t)H.nv * 4, hipMemcpyDeviceToHost));
+        p += (size_t)H.nv * 4;
+    }
+    HIPCHK(hipMemcpy(p, c->ligi + R.loff, (size_t)H.nlig * 4, hipMemcpyDeviceToHost));
+    p += (size_t)H.nlig * 4;
+    double pd[12];
+    int32_t pi[4] = {0, 0, 0, 0};
+    for (int k = 0; k < 3; ++k) {
+        pd[k] = R.para[k];
+        pd[3 + k] = R.paraV[k];
+        pd[6 + k] = R.paraft[k];
+        pd[9 + k] = R.parafdt[k];
+    }
+    pi[0] = R.paradx;
+    pi[1] = R.parady;
+    pi[2] = R.paraid;
+    std::memcpy(p, pd, sizeof pd);
+    p += sizeof pd;
+    std::memcpy(p, pi, sizeof pi);
+    p += sizeof pi;
+    std::vector<int32_t> off(H.ncell + 1);
+    HIPCHK(hipMemcpy(off.data(), c->lloff + R.coff1, (H.ncell + 1) * 4, hipMemcpyDeviceToHost));
+    const int32_t o0 = off[0];
+    for (int32_t &o : off) o -= o0;
+    std::memcpy(p, off.data(), (H.ncell + 1) * 4);
+    p += (H.ncell + 1) * 4;
+    if (llen > 0) HIPCHK(hipMemcpy(p, c->llent + R.llent_off + o0, llen * 4, hipMemcpyDeviceToHost));
+    p += llen * 4;
+    return (int64_t)(p - (char *)buf);
+}
+
+int irbc_checkpoint_load(irbc_ctx *c, int r, const void *buf, int64_t n)
+{
+    CHECK_R(c, r);
+    if (!buf || n < (int64_t)sizeof(CkptHeader)) return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: short buffer");
+    CkptHeader H;
+    std::memcpy(&H, buf, sizeof H);
+    const HostRep &h = c->hr[r];
+    if (std::memcmp(H.magic, "IRBCCKP1", 8) != 0 || H.version != 1)
+        return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: not a checkpoint of this format");
+    if (!h.mesh) return fail(IRBC_ERR_STATE, "irbc_checkpoint_load: load the replica's mesh first");
+    if (H.ns != h.spec.n_solvent || H.nv != h.spec.n_vertices || H.ne != h.spec.n_edges ||
+        H.nt != h.spec.n_triangles || H.ncell != c->rep[r].ncell || H.nlig != c->rep[r].nlig)
+        return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: replica %d has other sizes than the checkpoint", r);
+    if (n < ckpt_bytes(c, r, H.llen)) return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: truncated buffer");
+    if (H.llen > c->rep[r].llent_cap) return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: cell lists exceed capacity");
+    int rc = c->pull(r);
+    if (rc) return rc;
+    DevRep &R = c->rep[r];
+    const char *p = (const char *)buf + sizeof H;
+    float *sf[] = {c->xa, c->ya, c->za, c->vxa, c->vya, c->vza};
+    for (int k = 0; k < 6; ++k) {
+        HIPCHK(hipMemcpy(sf[k] + R.poff, p, H.ns * 4, hipMemcpyHostToDevice));
+        p += H.ns * 4;
+    }
+    HIPCHK(hipMemcpy(c->ida + R.poff, p, H.ns * 4, hipMemcpyHostToDevice));
+    p += H.ns * 4;
+    double *vd[] = {c->mx, c->my, c->mz, c->mvx, c->mvy, c->mvz, c->mfx, c->mfy, c->mfz};
+    for (int k = 0; k < 9; ++k) {
+        HIPCHK(hipMemcpy(vd[k] + R.voff, p, (size_t)H.nv * 8, hipMemcpyHostToDevice));
+        p += (size_t)H.nv * 8;
+    }
+    int32_t *vi[] = {c->mix, c->miy, c->mpid, c->bondi, c->bondl, c->knob};
+    for (int k = 0; k < 6; ++k) {
+        HIPCHK(hipMemcpy(vi[k] + R.voff, p, (size_t)H.nv * 4, hipMemcpyHostToDevice));
+        p += (size_t)H.nv * 4;
+    }
+    HIPCHK(hipMemcpy(c->ligi + R.loff, p, (size_t)H.nlig * 4, hipMemcpyHostToDevice));
+    p += (size_t)H.nlig * 4;
+    double pd[12];
+    int32_t pi[4];
+    std::memcpy(pd, p, sizeof pd);
+    p += sizeof pd;
+    std::memcpy(pi, p, sizeof pi);
+    p += sizeof pi;
+    for (int k = 0; k < 3; ++k) {
+        R.para[k] = pd[k];
+        R.paraV[k] = pd[3 + k];
+        R.paraft[k] = pd[6 + k];
+        R.parafdt[k] = pd[9 + k];
+    }
+    R.paradx = pi[0];
+    R.parady = pi[1];
+    R.paraid = pi[2];
+    std::vector<int32_t> off(H.ncell + 1);
+    std::memcpy(off.data(), p, (H.ncell + 1) * 4);
+    p += (H.ncell + 1) * 4;
+    HIPCHK(hipMemcpy(c->lloff + R.coff1, off.data(), (H.ncell + 1) * 4, hipMemcpyHostToDevice));
+    if (H.llen > 0) HIPCHK(hipMemcpy(c->llent + R.llent_off, p, H.llen * 4, hipMemcpyHostToDevice));
+    /* the particle order of the checkpoint, no pending rotation */
+    std::vector<int32_t> zero(H.ns, 0);
+    HIPCHK(hipMemcpy(c->skey2 + R.poff, zero.data(), H.ns * 4, hipMemcpyHostToDevice));
+    R.cur = 0;
+    R.rotate_pending = 0;
+    R.seed = H.seed;
+    R.tapes = H.tapes;
+    R.status = 0;
+    R.near_count[0] = R.near_count[1] = R.defer_count[0] = R.defer_count[1] = 0;
+    HostRep &hm = c->hr[r];
+    hm.spec.seed = H.seed;
+    hm.step = H.step;
+    hm.colsteps = H.colsteps;
+    hm.upd = H.upd;
+    hm.solvent = true;
+    top_of_step(hm.spec.params, H.step, H.colsteps, H.upd, R);
+    R.wall2[0] = R.wall2[1] = H.wallspeed;
+    return c->push(r);
+}
+
 /* ---------------- the time loop ---------------- */
+
+/* Launches of one MD step.  Two chains: the solvent chain (collision, cell
+ * lists, streaming, bounce-back) on c->st and the membrane chain on c->st2.
+ * Streaming of step s only reads the triangle kinematics of step s, so the
+ * membrane of step s (which writes those of step s+1 into the other
+ * parity's buffer) runs beside it.  The step's last act is the solvent
+ * stream waiting for the membrane: the collision of the next step moves
+ * the vertices.  `rec` brackets the phases with timing events. */
+static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec)
+{
+    int rc;
+    const int mem_phases = PH_ALL & ~PH_RESET_NEAR; /* k_stream clears the other parity's queue */
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (col) {
+        if (rec) { e0 = c->event(); hipEventRecord(e0, c->st); }
+        rc = launch_collision(c, 0, c->diag_on ? 1 : 0);
+        if (rc) return rc;
+        if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[1].push_back({e0, e1}); }
+    }
+    if (ll) {
+        rc = launch_llist(c, 0);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(c->ev_pre, c->st));
+    HIPCHK(hipStreamWaitEvent(c->st2, c->ev_pre, 0));
+    if (rec) { e0 = c->event(); hipEventRecord(e0, c->st2); }
+    rc = launch_membrane(c, mem_phases, c->st2);
+    if (rc) return rc;
+    if (rec) { e1 = c->event(); hipEventRecord(e1, c->st2); c->pend[2].push_back({e0, e1}); }
+    HIPCHK(hipEventRecord(c->ev_mem, c->st2));
+    if (rec) {
+        /* events bound to the dispatch itself: the kernel's own duration */
+        e0 = c->event();
+        e1 = c->event();
+        hipExtLaunchKernelGGL(k_stream, dim3((unsigned)c->z.tiles), dim3(IRBC_WG), 0, c->st, e0, e1, 0,
+                              c->d_rep, (const int32_t *)c->d_tile_rep,
+                              (const int32_t *)c->skey2, (const double *)c->ctab,
+                              (const int32_t *)c->lloff, c->nearq, c->par);
+        c->pend[0].push_back({e0, e1});
+        e0 = c->event();
+        hipEventRecord(e0, c->st);
+    } else {
+        k_stream<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab,
+                                                             c->lloff, c->nearq, c->par);
+    }
+    rc = launch_near(c);
+    if (rc) return rc;
+    if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[3].push_back({e0, e1}); }
+    HIPCHK(hipStreamWaitEvent(c->st, c->ev_mem, 0));
+    LAUNCHCHK();
+    return IRBC_OK;
+}
+
+/* collide / list-update flags of the next step of every replica, then
+ * the host mirror advanced by one step */
+static void next_flags(irbc_ctx *c, std::vector<HostRep> &hr, bool &col, bool &ll)
+{
+    col = ll = false;
+    for (int r = 0; r < c->nrep; ++r) {
+        col |= host_collides(hr[r].spec.params, hr[r]);
+        ll |= host_llist(hr[r].spec.params, hr[r]);
+        advance_host(hr[r].spec.params, hr[r]);
+    }
+}
+
+/* A chunk of steps captured once as a HIP graph and replayed whenever the
+ * same chunk shape comes again: key = the collide and list-update flags of
+ * each step, the starting parity and the launch options. */
+static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
+{
+    const uint64_t key = flags | ((uint64_t)(c->par & 1) << 48) | ((uint64_t)n << 50) |
+                         ((uint64_t)(c->mem_fused & 1) << 58) | ((uint64_t)(c->diag_on & 1) << 59);
+    auto it = c->graphs.find(key);
+    if (it == c->graphs.end()) {
+        const int par0 = c->par;
+        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        int rc = IRBC_OK;
+        for (int k = 0; k < n && rc == IRBC_OK; ++k) {
+            rc = launch_step(c, (flags >> (2 * k)) & 1, (flags >> (2 * k + 1)) & 1, false);
+            c->par ^= 1;
+        }
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(c->st, &g);
+        c->par = par0;
+        if (rc) {
+            if (g) hipGraphDestroy(g);
+            return rc;
+        }
+        if (ec != hipSuccess) return fail(IRBC_ERR_HIP, "graph capture: %s", hipGetErrorString(ec));
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) return fail(IRBC_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ei));
+        if (c->graphs.size() >= 64) { /* bounded cache: drop everything */
+            for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
+            c->graphs.clear();
+        }
+        it = c->graphs.emplace(key, ex).first;
+    }
+    HIPCHK(hipGraphLaunch(it->second, c->st));
+    if (n & 1) c->par ^= 1;
+    return IRBC_OK;
+}
 
 int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
 {
     if (!c || nsteps < 0) return fail(IRBC_ERR_ARG, "irbc_md_steps: bad arguments");
     int rc = ready(c, true);
     if (rc) return rc;
+    if (nsteps == 0) return IRBC_OK;
     k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 1);
     rc = launch_tri_kin(c);
     if (rc) return rc;
-    /* Two chains per step: the solvent chain (collision, cell lists,
-     * streaming, bounce-back) on c->st and the membrane chain on c->st2.
-     * Streaming of step s only reads the triangle kinematics of step s, so
-     * the membrane of step s (which writes those of step s+1 into the other
-     * parity's buffer) runs beside it.  The collision of step s+1 waits for
-     * the membrane of step s: it moves the vertices. */
-    const int mem_phases = PH_ALL & ~PH_RESET_NEAR; /* k_stream clears the other parity's queue */
+    if (c->use_graphs && !c->prof) {
+        const int CH = IRBC_GRAPH_STEPS;
+        for (int64_t s = 0; s < nsteps;) {
+            const int n = (int)std::min<int64_t>(CH, nsteps - s);
+            uint64_t flags = 0;
+            for (int k = 0; k < n; ++k) {
+                bool col, ll;
+                next_flags(c, c->hr, col, ll);
+                flags |= (uint64_t)col << (2 * k) | (uint64_t)ll << (2 * k + 1);
+            }
+            rc = run_graph_chunk(c, n, flags);
+            if (rc) return rc;
+            s += n;
+        }
+        return IRBC_OK;
+    }
     for (int64_t s = 0; s < nsteps; ++s) {
-        bool col = false, ll = false;
-        for (int r = 0; r < c->nrep; ++r) {
-            col |= host_collides(c->hr[r].spec.params, c->hr[r]);
-            ll |= host_llist(c->hr[r].spec.params, c->hr[r]);
-        }
-        if (s > 0) HIPCHK(hipStreamWaitEvent(c->st, c->ev_mem, 0));
-        hipEvent_t e0 = nullptr, e1 = nullptr;
+        bool col, ll;
+        next_flags(c, c->hr, col, ll);
         const bool rec = c->prof && (c->prof_step++ % c->prof_every) == 0;
-        if (col) {
-            if (rec) { e0 = c->event(); hipEventRecord(e0, c->st); }
-            rc = launch_collision(c, 0, c->diag_on ? 1 : 0);
-            if (rc) return rc;
-            if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[1].push_back({e0, e1}); }
-        }
-        if (ll) {
-            rc = launch_llist(c, 0);
-            if (rc) return rc;
-        }
-        HIPCHK(hipEventRecord(c->ev_pre, c->st));
-        HIPCHK(hipStreamWaitEvent(c->st2, c->ev_pre, 0));
-        if (rec) { e0 = c->event(); hipEventRecord(e0, c->st2); }
-        rc = launch_membrane(c, mem_phases, c->st2);
+        rc = launch_step(c, col, ll, rec);
         if (rc) return rc;
-        if (rec) { e1 = c->event(); hipEventRecord(e1, c->st2); c->pend[2].push_back({e0, e1}); }
-        HIPCHK(hipEventRecord(c->ev_mem, c->st2));
-        if (rec) {
-            /* events bound to the dispatch itself: the kernel's own duration */
-            e0 = c->event();
-            e1 = c->event();
-            hipExtLaunchKernelGGL(k_stream, dim3((unsigned)c->z.tiles), dim3(IRBC_WG), 0, c->st, e0, e1, 0,
-                                  c->d_rep, (const int32_t *)c->d_tile_rep,
-                                  (const int32_t *)c->skey2, (const double *)c->ctab,
-                                  (const int32_t *)c->lloff, c->nearq, c->par);
-            c->pend[0].push_back({e0, e1});
-            e0 = c->event();
-            hipEventRecord(e0, c->st);
-        } else {
-            k_stream<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab,
-                                                                 c->lloff, c->nearq, c->par);
-        }
-        rc = launch_near(c);
-        if (rc) return rc;
-        if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[3].push_back({e0, e1}); }
-        LAUNCHCHK();
-        for (int r = 0; r < c->nrep; ++r) advance_host(c->hr[r].spec.params, c->hr[r]);
         c->par ^= 1;
         if (c->prof && c->pend[0].size() >= 2048) {
             rc = c->flush_prof();
             if (rc) return rc;
         }
     }
-    /* later work on c->st sees the last membrane chain */
-    if (nsteps > 0) HIPCHK(hipStreamWaitEvent(c->st, c->ev_mem, 0));
     return IRBC_OK;
 }
 
@@ -1333,6 +1615,9 @@ int irbc_set_option(irbc_ctx *c, int option, int value)
     switch (option) {
     case IRBC_OPT_MEMBRANE_FUSED:
         c->mem_fused = value != 0;
+        return IRBC_OK;
+    case IRBC_OPT_GRAPHS:
+        c->use_graphs = value != 0;
         return IRBC_OK;
     default:
         return fail(IRBC_ERR_ARG, "irbc_set_option: unknown option %d", option);

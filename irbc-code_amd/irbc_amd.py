@@ -26,9 +26,11 @@ EXPORTS = (
     "irbc_membrane_forces", "irbc_bond_step", "irbc_get_cells", "irbc_get_llist",
     "irbc_get_pairs", "irbc_md_steps", "irbc_set_diagnostics", "irbc_profile",
     "irbc_profile_read", "irbc_set_series", "irbc_get_series", "irbc_query", "irbc_set_option",
+    "irbc_checkpoint_bytes", "irbc_checkpoint_save", "irbc_checkpoint_load",
 )
 
 OPT_MEMBRANE_FUSED = 1
+OPT_GRAPHS = 2
 
 SERIES_FIELDS = ("step", "x", "y", "z", "vx", "vy", "vz", "bonds")
 
@@ -132,6 +134,11 @@ def load():
     L.irbc_get_series.restype = C.c_int64
     L.irbc_get_series.argtypes = [C.c_void_p, C.c_int, _d, C.c_int64]
     L.irbc_query.argtypes = [C.c_void_p]
+    L.irbc_checkpoint_bytes.restype = C.c_int64
+    L.irbc_checkpoint_bytes.argtypes = [C.c_void_p, C.c_int]
+    L.irbc_checkpoint_save.restype = C.c_int64
+    L.irbc_checkpoint_save.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
+    L.irbc_checkpoint_load.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
     _lib = L
     return L
 
@@ -373,6 +380,18 @@ class Batch:
         out = np.zeros((max(n, 1), len(SERIES_FIELDS)))
         _chk(self.L.irbc_get_series(self.h, r, _ptr(out, _d), n))
         return out[:n]
+
+    def checkpoint(self, r=0):
+        """The replica's resumable state as bytes."""
+        n = _chk(self.L.irbc_checkpoint_bytes(self.h, r))
+        buf = C.create_string_buffer(n)
+        m = _chk(self.L.irbc_checkpoint_save(self.h, r, buf, n))
+        return buf.raw[:m]
+
+    def restore(self, r, data):
+        """Load a checkpoint into replica r (its mesh must be set)."""
+        buf = C.create_string_buffer(bytes(data), len(data))
+        _chk(self.L.irbc_checkpoint_load(self.h, r, buf, len(data)))
 
     def done(self):
         """True when all launched work has finished (non-blocking)."""

@@ -14,6 +14,7 @@
 #include "oracle.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -993,6 +994,20 @@ double orc_solve(double aa, double bb, double cc, double dtmd)
     return tt;
 }
 
+/* diagnostics of the bounce-back contact times (not part of the
+ * reference): hits, hits at a negative contact time, the smallest contact
+ * time of a hit, plane crossings examined */
+static std::atomic<int64_t> g_hits{0}, g_hits_neg{0}, g_cross{0};
+static std::atomic<double> g_tmin{0.0};
+
+void orc_hit_stats(double *out)
+{
+    out[0] = (double)g_hits.exchange(0);
+    out[1] = (double)g_hits_neg.exchange(0);
+    out[2] = g_tmin.exchange(0.0);
+    out[3] = (double)g_cross.exchange(0);
+}
+
 double orc_reflection(orc *o, int64_t i)
 {
     /* reflectionFed(i, 0) (Src/solfuncs.cpp:400-598): bounce-back off the
@@ -1109,7 +1124,14 @@ double orc_reflection(orc *o, int64_t i)
             double denom = (a12 * a22) - (a1a2 * a1a2);
             double eta = ((pa1 * a22) - (pa2 * a1a2)) / denom;
             double zeta = ((pa2 * a12) - (pa1 * a1a2)) / denom;
+            g_cross.fetch_add(1, std::memory_order_relaxed);
             if ((eta >= 0) && (zeta >= 0) && (eta + zeta <= 1) && (tt < (dtmd - rstep))) {
+                g_hits.fetch_add(1, std::memory_order_relaxed);
+                if (tt < 0) {
+                    g_hits_neg.fetch_add(1, std::memory_order_relaxed);
+                    double cur = g_tmin.load();
+                    while (tt < cur && !g_tmin.compare_exchange_weak(cur, tt)) {}
+                }
                 fulldt += tt;
                 double vd[3];
                 vd[0] = (1.0 - eta - zeta) * o->mvx[p1] + eta * o->mvx[p2] + zeta * o->mvx[p3];

@@ -128,6 +128,9 @@ void orc_md_step(orc *o);
 /* init of the README run (maincode.cpp:58-446) from raw mesh data; edges and
  * triangles 0-based.  stats[8] = mean edge, SD, Sarea0, Volume0, shear,
  * Young's, Poisson, knobs selected. */
+/* bounce-back diagnostics since the last call: out[4] = hits, hits at a
+ * negative contact time, smallest such time, plane crossings examined */
+void orc_hit_stats(double *out);
 int orc_init_readme(orc *o, int32_t seed, int64_t ns, int nv, const double *xyz_raw, int ne,
                     const int32_t *edges, int nt, const int32_t *tris, int nknobs, double *stats);
 
