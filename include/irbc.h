@@ -148,15 +148,31 @@ int64_t irbc_get_pairs(irbc_ctx *ctx, int r, int32_t *cnt, int32_t *lig, int64_t
 /* ---- the time loop ---- */
 int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
 /* launch-shape options (no counterpart in the reference):
- * IRBC_OPT_MEMBRANE_FUSED  1 (default): the membrane part of a step is one
- *   workgroup per replica; 0: five grid-wide kernels.  Both give the same
+ * IRBC_OPT_MEMBRANE_FUSED  1: the membrane part of a step is one workgroup
+ *   per replica; 0 (default): five grid-wide kernels.  Both give the same
  *   forces up to the summation order of the global area and volume. */
 #define IRBC_OPT_MEMBRANE_FUSED 1
-/* IRBC_OPT_GRAPHS  1: irbc_md_steps captures chunks of 10 steps as HIP
- *   graphs (one per chunk shape) and replays them; 0 (default): launches
+/* IRBC_OPT_GRAPHS  1 (default): irbc_md_steps captures chunks of 10 steps
+ *   as HIP graphs (one per chunk shape) and replays them; 0: launches
  *   kernel by kernel.  Ignored while event profiling is on. */
 #define IRBC_OPT_GRAPHS 2
+/* IRBC_OPT_NEAR_MERGED  1 (default): bounce-back off the membrane in one
+ *   kernel; 0: a plane-crossing pass, then a hit pass over the particles
+ *   that cross a plane.  Same results. */
+#define IRBC_OPT_NEAR_MERGED 3
+/* IRBC_OPT_XCD_CHUNK  1 (default): the binning, scatter and cell kernels
+ *   hand each of the 8 XCDs a contiguous eighth of the tiles; 0: plain
+ *   round-robin dealing.  Same results. */
+#define IRBC_OPT_XCD_CHUNK 4
 int irbc_set_option(irbc_ctx *ctx, int option, int value);
+/* Times `steps` MD steps under each of the 16 combinations of the options
+ * above (membrane shape, near shape, graphs, XCD chunking) on this
+ * context's own replicas, then keeps the fastest.  The state is snapshot on
+ * the device first and restored afterwards, so the trajectory is not
+ * advanced.  report (may be NULL) receives 16 ms-per-step figures, index =
+ * fused + 2 near_merged + 4 graphs + 8 xcd_chunk.  Returns the chosen
+ * index or < 0.  Needs free device memory for one copy of the context. */
+int irbc_autotune(irbc_ctx *ctx, int32_t steps, double *report);
 /* keep per-cell results (irbc_get_cells) also for collisions run inside
  * irbc_md_steps; costs ~56 B per cell per collision */
 int irbc_set_diagnostics(irbc_ctx *ctx, int on);

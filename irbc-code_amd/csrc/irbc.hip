@@ -195,8 +195,10 @@ struct irbc_ctx {
     hipStream_t st2 = nullptr;  /* membrane chain, overlapped with the solvent's streaming */
     hipEvent_t ev_pre = nullptr, ev_mem = nullptr;
     int par = 0;                /* parity of the current step: picks the double-buffered state */
-    int mem_fused = 1;          /* membrane step as one workgroup per replica (1) or five grid kernels (0) */
-    int use_graphs = 0;         /* irbc_md_steps replays captured chunks of IRBC_GRAPH_STEPS steps */
+    int mem_fused = 0;          /* membrane step as one workgroup per replica (1) or five grid kernels (0) */
+    int use_graphs = 1;         /* irbc_md_steps replays captured chunks of IRBC_GRAPH_STEPS steps */
+    int near_merged = 1;        /* near-membrane streaming in one kernel (1) or plane pass + hit pass (0) */
+    int xcd_chunk = 1;          /* sort and cell kernels hand each XCD a contiguous eighth of the tiles */
     std::map<uint64_t, hipGraphExec_t> graphs;
     int nrep = 0;
     std::vector<HostRep> hr;
@@ -204,6 +206,7 @@ struct irbc_ctx {
     Sizes z;
     int64_t bytes = 0;
     std::vector<void *> allocs;
+    std::vector<size_t> alloc_bytes; /* size of each entry of allocs */
 
     DevRep *d_rep = nullptr;
     int32_t *d_tile_rep = nullptr, *d_ctile_rep = nullptr, *d_rep_first_blk = nullptr;
@@ -251,6 +254,7 @@ struct irbc_ctx {
         if (hipMemsetAsync(q, 0, (size_t)n * sizeof(T), st) != hipSuccess)
             return fail(IRBC_ERR_HIP, "hipMemsetAsync failed");
         allocs.push_back(q);
+        alloc_bytes.push_back((size_t)n * sizeof(T));
         bytes += n * (int64_t)sizeof(T);
         p = (T *)q;
         return IRBC_OK;
@@ -1020,6 +1024,7 @@ static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t 
 }
 
 #define IRBC_GRAPH_STEPS 10 /* steps per captured chunk: one collision period */
+#define IRBC_TUNE_CANDIDATES 16 /* membrane shape x near shape x graphs x XCD chunking */
 
 /* grid for a kernel that maps blocks through xcd_tile */
 static unsigned xcd_grid(unsigned n) { return IRBC_NXCD * ((n + IRBC_NXCD - 1) / IRBC_NXCD); }
@@ -1028,15 +1033,17 @@ static int launch_collision(irbc_ctx *c, int staged, int diag)
 {
     const unsigned T = (unsigned)c->z.tiles, CT = (unsigned)c->z.ctiles;
     if (staged) k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 0);
-    k_bin<<<xcd_grid(T), IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, staged, c->skey, c->srank,
-                                              c->ccount);
+    const int xc = c->xcd_chunk;
+    k_bin<<<xc ? xcd_grid(T) : T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, xc, staged, c->skey,
+                                                       c->srank, c->ccount);
     int rc = launch_scan(c, staged, 0, c->ccount, c->cstart);
     if (rc) return rc;
-    k_scatter<<<xcd_grid(T), IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, staged, c->skey, c->srank,
-                                                  c->cstart, c->skey2);
+    k_scatter<<<xc ? xcd_grid(T) : T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, xc, staged, c->skey,
+                                                           c->srank, c->cstart, c->skey2);
     MemDev M = c->mem(c->par);
     k_vbin<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, staged, M, c->vhead, c->vnext);
-    k_cell<<<xcd_grid(CT), IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, staged, c->cstart, c->vhead,
+    k_cell<<<xc ? xcd_grid(CT) : CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, xc, staged, c->cstart,
+                                                         c->vhead,
                                                 c->vnext,
                                       c->mvx, c->mvy, c->mvz, c->cell_u, c->wall_z, c->mem_z, c->ctab,
                                       diag, c->diag, c->par);
@@ -1072,6 +1079,12 @@ static unsigned near_blocks(const irbc_ctx *c)
 static int launch_near(irbc_ctx *c)
 {
     const double *tk = c->tri_kin + (c->par & 1) * c->tri_stride();
+    if (c->near_merged) {
+        k_near<<<dim3(near_blocks(c), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(c->d_rep, c->lloff, c->llent,
+                                                                             c->nearq, tk, c->par);
+        LAUNCHCHK();
+        return IRBC_OK;
+    }
     k_near_plane<<<dim3(near_blocks(c), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
         c->d_rep, c->lloff, c->llent, c->nearq, c->deferq, tk, c->par);
     k_near_hit<<<dim3(std::max(16u, near_blocks(c) / 8u), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
@@ -1257,6 +1270,102 @@ int64_t irbc_get_pairs(irbc_ctx *c, int r, int32_t *cnt, int32_t *lig, int64_t c
         }
     }
     return t;
+}
+
+/* ---------------- launch-shape autotuner ---------------- */
+
+/* The whole device state of a context (every allocation) and the host
+ * mirrors, copied on the device so that trial steps can be undone. */
+struct Snapshot {
+    std::vector<void *> buf;
+    std::vector<HostRep> hr;
+    std::vector<DevRep> rep;
+    int par = 0;
+};
+
+static int snap_take(irbc_ctx *c, Snapshot &S)
+{
+    HIPCHK(c->sync_both());
+    if (S.buf.empty()) {
+        for (size_t i = 0; i < c->allocs.size(); ++i) {
+            void *q = nullptr;
+            if (hipMalloc(&q, c->alloc_bytes[i]) != hipSuccess) {
+                for (void *b : S.buf) hipFree(b);
+                S.buf.clear();
+                return fail(IRBC_ERR_OOM, "autotune: no room for a state snapshot");
+            }
+            S.buf.push_back(q);
+        }
+    }
+    for (size_t i = 0; i < c->allocs.size(); ++i)
+        HIPCHK(hipMemcpyAsync(S.buf[i], c->allocs[i], c->alloc_bytes[i], hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    S.hr = c->hr;
+    S.rep = c->rep;
+    S.par = c->par;
+    return IRBC_OK;
+}
+
+static int snap_restore(irbc_ctx *c, const Snapshot &S)
+{
+    HIPCHK(c->sync_both());
+    for (size_t i = 0; i < c->allocs.size(); ++i)
+        HIPCHK(hipMemcpyAsync(c->allocs[i], S.buf[i], c->alloc_bytes[i], hipMemcpyDeviceToDevice, c->st));
+    HIPCHK(hipStreamSynchronize(c->st));
+    c->hr = S.hr;
+    c->rep = S.rep;
+    c->par = S.par;
+    return IRBC_OK;
+}
+
+int irbc_autotune(irbc_ctx *c, int32_t steps, double *report)
+{
+    if (!c || steps < 1) return fail(IRBC_ERR_ARG, "irbc_autotune: bad arguments");
+    int rc = ready(c, true);
+    if (rc) return rc;
+    if (c->prof) return fail(IRBC_ERR_STATE, "irbc_autotune: switch event profiling off first");
+    Snapshot S;
+    rc = snap_take(c, S);
+    if (rc) return rc;
+    double best = 1e300;
+    int best_k = 0;
+    for (int k = 0; k < IRBC_TUNE_CANDIDATES; ++k) {
+        c->mem_fused = (k >> 0) & 1;
+        c->near_merged = (k >> 1) & 1;
+        c->use_graphs = (k >> 2) & 1;
+        c->xcd_chunk = (k >> 3) & 1;
+        if (!rc) rc = snap_restore(c, S);
+        if (!rc) rc = irbc_md_steps(c, steps); /* warm-up; also captures graphs */
+        double ms = 0.0;
+        if (!rc) {
+            rc = snap_restore(c, S);
+            hipEvent_t e0 = c->event(), e1 = c->event();
+            if (!rc) rc = hipEventRecord(e0, c->st) == hipSuccess ? IRBC_OK : IRBC_ERR_HIP;
+            if (!rc) rc = irbc_md_steps(c, steps);
+            if (!rc) rc = hipEventRecord(e1, c->st) == hipSuccess ? IRBC_OK : IRBC_ERR_HIP;
+            if (!rc) rc = c->wait_all();
+            float f = 0.f;
+            if (!rc && hipEventElapsedTime(&f, e0, e1) != hipSuccess) rc = fail(IRBC_ERR_HIP, "autotune timing");
+            ms = f;
+            c->ev_pool.push_back(e0);
+            c->ev_pool.push_back(e1);
+        }
+        if (rc) break;
+        if (report) report[k] = ms / steps;
+        if (ms < best) {
+            best = ms;
+            best_k = k;
+        }
+    }
+    int rc2 = snap_restore(c, S);
+    for (void *b : S.buf) hipFree(b);
+    if (rc) return rc;
+    if (rc2) return rc2;
+    c->mem_fused = (best_k >> 0) & 1;
+    c->near_merged = (best_k >> 1) & 1;
+    c->use_graphs = (best_k >> 2) & 1;
+    c->xcd_chunk = (best_k >> 3) & 1;
+    return best_k;
 }
 
 /* ---------------- checkpoint / resume ---------------- */
@@ -1536,7 +1645,8 @@ static void next_flags(irbc_ctx *c, std::vector<HostRep> &hr, bool &col, bool &l
 static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
 {
     const uint64_t key = flags | ((uint64_t)(c->par & 1) << 48) | ((uint64_t)n << 50) |
-                         ((uint64_t)(c->mem_fused & 1) << 58) | ((uint64_t)(c->diag_on & 1) << 59);
+                         ((uint64_t)(c->mem_fused & 1) << 58) | ((uint64_t)(c->diag_on & 1) << 59) |
+                         ((uint64_t)(c->near_merged & 1) << 60) | ((uint64_t)(c->xcd_chunk & 1) << 61);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         const int par0 = c->par;
@@ -1618,6 +1728,12 @@ int irbc_set_option(irbc_ctx *c, int option, int value)
         return IRBC_OK;
     case IRBC_OPT_GRAPHS:
         c->use_graphs = value != 0;
+        return IRBC_OK;
+    case IRBC_OPT_NEAR_MERGED:
+        c->near_merged = value != 0;
+        return IRBC_OK;
+    case IRBC_OPT_XCD_CHUNK:
+        c->xcd_chunk = value != 0;
         return IRBC_OK;
     default:
         return fail(IRBC_ERR_ARG, "irbc_set_option: unknown option %d", option);

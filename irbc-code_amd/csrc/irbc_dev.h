@@ -175,10 +175,12 @@ __device__ __forceinline__ int32_t cell_of(int fx, int fy, int fz, int Lx, int L
  * of xcd_grid(n) blocks mapped through xcd_tile hands each XCD one
  * contiguous eighth of the n logical tiles, so writes that land in the same
  * cache lines (neighbouring cells of a sort) come from one XCD's L2.
- * Blocks with xcd_tile(n) >= n do nothing. */
+ * Blocks with xcd_tile(n) >= n do nothing.  With chunked = 0 the block
+ * index is the tile (a grid of exactly n blocks). */
 #define IRBC_NXCD 8
-__device__ __forceinline__ int xcd_tile(int n)
+__device__ __forceinline__ int xcd_tile(int n, int chunked)
 {
+    if (!chunked) return (int)blockIdx.x;
     const int per = (n + IRBC_NXCD - 1) / IRBC_NXCD;
     return (int)(blockIdx.x % IRBC_NXCD) * per + (int)(blockIdx.x / IRBC_NXCD);
 }

@@ -26,11 +26,13 @@ EXPORTS = (
     "irbc_membrane_forces", "irbc_bond_step", "irbc_get_cells", "irbc_get_llist",
     "irbc_get_pairs", "irbc_md_steps", "irbc_set_diagnostics", "irbc_profile",
     "irbc_profile_read", "irbc_set_series", "irbc_get_series", "irbc_query", "irbc_set_option",
-    "irbc_checkpoint_bytes", "irbc_checkpoint_save", "irbc_checkpoint_load",
+    "irbc_checkpoint_bytes", "irbc_checkpoint_save", "irbc_checkpoint_load", "irbc_autotune",
 )
 
 OPT_MEMBRANE_FUSED = 1
 OPT_GRAPHS = 2
+OPT_NEAR_MERGED = 3
+OPT_XCD_CHUNK = 4
 
 SERIES_FIELDS = ("step", "x", "y", "z", "vx", "vy", "vz", "bonds")
 
@@ -134,6 +136,7 @@ def load():
     L.irbc_get_series.restype = C.c_int64
     L.irbc_get_series.argtypes = [C.c_void_p, C.c_int, _d, C.c_int64]
     L.irbc_query.argtypes = [C.c_void_p]
+    L.irbc_autotune.argtypes = [C.c_void_p, C.c_int32, _d]
     L.irbc_checkpoint_bytes.restype = C.c_int64
     L.irbc_checkpoint_bytes.argtypes = [C.c_void_p, C.c_int]
     L.irbc_checkpoint_save.restype = C.c_int64
@@ -392,6 +395,13 @@ class Batch:
         """Load a checkpoint into replica r (its mesh must be set)."""
         buf = C.create_string_buffer(bytes(data), len(data))
         _chk(self.L.irbc_checkpoint_load(self.h, r, buf, len(data)))
+
+    def autotune(self, steps=20):
+        """Pick the fastest launch options for this batch; returns
+        (chosen index, ms per step of each of the 16 candidates)."""
+        rep = np.zeros(16)
+        k = _chk(self.L.irbc_autotune(self.h, int(steps), _ptr(rep, _d)))
+        return k, rep
 
     def done(self):
         """True when all launched work has finished (non-blocking)."""

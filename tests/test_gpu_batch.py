@@ -121,3 +121,44 @@ def test_fused_membrane_matches_grid_kernels():
             np.testing.assert_array_equal(v1[k], v0[k])
         assert abs(g1["Sarea"] - g0["Sarea"]) <= 1e-12 * g0["Sarea"]
         assert abs(g1["Volume"] - g0["Volume"]) <= 1e-12 * g0["Volume"]
+
+
+@pytest.mark.gpu
+def test_autotune_leaves_the_trajectory_alone():
+    """The autotuner's trial steps are undone: a batch tuned and then run
+    equals a batch run with the chosen options from the start."""
+    ds = _mix()
+    b1 = en.batch(ds)
+    b1.md_steps(7)
+    k, rep = b1.autotune(12)
+    assert 0 <= k < 16 and np.all(rep > 0)
+    b1.md_steps(25)
+    b1.sync()
+    b2 = en.batch(ds)
+    for opt, bit in ((ia.OPT_MEMBRANE_FUSED, 0), (ia.OPT_NEAR_MERGED, 1), (ia.OPT_GRAPHS, 2),
+                     (ia.OPT_XCD_CHUNK, 3)):
+        b2.set_option(opt, (k >> bit) & 1)
+    b2.md_steps(32)
+    b2.sync()
+    for r in range(len(ds)):
+        s1, s2 = _snapshot(b1, r), _snapshot(b2, r)
+        for key in s1:
+            np.testing.assert_array_equal(s1[key], s2[key], err_msg="replica %d %s" % (r, key))
+
+
+@pytest.mark.gpu
+def test_launch_options_do_not_change_results():
+    """Near-membrane shape and XCD chunking are pure launch choices."""
+    ds = _mix()
+    snaps = []
+    for near, xcd in ((1, 1), (0, 1), (1, 0)):
+        b = en.batch(ds)
+        b.set_option(ia.OPT_NEAR_MERGED, near)
+        b.set_option(ia.OPT_XCD_CHUNK, xcd)
+        b.md_steps(31)
+        b.sync()
+        snaps.append([_snapshot(b, r) for r in range(len(ds))])
+    for other in snaps[1:]:
+        for r in range(len(ds)):
+            for key in snaps[0][r]:
+                np.testing.assert_array_equal(snaps[0][r][key], other[r][key], err_msg="replica %d %s" % (r, key))
