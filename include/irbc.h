@@ -164,13 +164,17 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
  *   hand each of the 8 XCDs a contiguous eighth of the tiles; 0: plain
  *   round-robin dealing.  Same results. */
 #define IRBC_OPT_XCD_CHUNK 4
+/* IRBC_OPT_STREAM4  1: bulk streaming with four consecutive particles per
+ *   lane (16-byte loads and stores); 0 (default): one per lane.  Same
+ *   results. */
+#define IRBC_OPT_STREAM4 5
 int irbc_set_option(irbc_ctx *ctx, int option, int value);
-/* Times `steps` MD steps under each of the 16 combinations of the options
- * above (membrane shape, near shape, graphs, XCD chunking) on this
+/* Times `steps` MD steps under each of the 32 combinations of the options
+ * above (membrane shape, near shape, graphs, XCD chunking, stream shape) on this
  * context's own replicas, then keeps the fastest.  The state is snapshot on
  * the device first and restored afterwards, so the trajectory is not
- * advanced.  report (may be NULL) receives 16 ms-per-step figures, index =
- * fused + 2 near_merged + 4 graphs + 8 xcd_chunk.  Returns the chosen
+ * advanced.  report (may be NULL) receives 32 ms-per-step figures, index =
+ * fused + 2 near_merged + 4 graphs + 8 xcd_chunk + 16 stream4.  Returns the chosen
  * index or < 0.  Needs free device memory for one copy of the context. */
 int irbc_autotune(irbc_ctx *ctx, int32_t steps, double *report);
 /* keep per-cell results (irbc_get_cells) also for collisions run inside

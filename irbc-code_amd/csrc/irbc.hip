@@ -179,7 +179,7 @@ int64_t bytes_of(const Sizes &z, int n)
     b += z.cells1 * (4 + 4);                              /* start, lloff */
     b += z.llent * 4;
     b += (int64_t)z.nv * (19 * 8 + 12 * 4 + IRBC_MAX_CAND * (4 + 8));
-    b += (int64_t)z.nt * IRBC_TRI_KIN * 8 + (int64_t)z.part * 16 + z.slots * 8;
+    b += 2 * (int64_t)z.nt * (IRBC_TRI_KIN * 8 + 16) + (int64_t)z.part * 16 + z.slots * 8;
     b += (int64_t)z.ne * (5 * 4 + 2 * 8) + (int64_t)z.nt * (3 * 4 + 2 * 8);
     b += z.scr * 24 + (int64_t)z.inc * 4 + (int64_t)(z.nv + n) * 4;
     b += (int64_t)z.nlig * 4 + (int64_t)(z.nligx + z.nligy) * 8;
@@ -199,6 +199,7 @@ struct irbc_ctx {
     int use_graphs = 1;         /* irbc_md_steps replays captured chunks of IRBC_GRAPH_STEPS steps */
     int near_merged = 1;        /* near-membrane streaming in one kernel (1) or plane pass + hit pass (0) */
     int xcd_chunk = 1;          /* sort and cell kernels hand each XCD a contiguous eighth of the tiles */
+    int stream4 = 0;            /* bulk streaming with four particles per lane */
     std::map<uint64_t, hipGraphExec_t> graphs;
     int nrep = 0;
     std::vector<HostRep> hr;
@@ -226,6 +227,7 @@ struct irbc_ctx {
     int32_t *mix, *miy, *mpid, *bondi, *bondl, *knob, *vnext;
     int32_t *e1, *e2, *es1, *es2, *iedge, *t1, *t2, *t3;
     double *l0, *kappap, *tarea, *tri_area, *wca, *slot, *upos, *part, *tri_kin;
+    float4 *tri_sph;
     int32_t *inc_off, *inc_ent;
     int32_t *ncand, *cand, *draw_off, *flag, *list;
     int32_t *ligi;
@@ -272,6 +274,7 @@ struct irbc_ctx {
         M.l0 = l0; M.kappap = kappap; M.tarea = tarea;
         M.tri_area = tri_area; M.wca = wca; M.slot = slot;
         M.upos = upos; M.part = part; M.tri_kin = tri_kin + (tbuf & 1) * tri_stride();
+        M.tri_sph = tri_sph + (tbuf & 1) * z.nt;
         M.inc_off = inc_off; M.inc_ent = inc_ent;
         M.ncand = ncand; M.cand = cand; M.draw_off = draw_off; M.flag = flag; M.list = list;
         M.ligi = ligi; M.ligx = ligx; M.ligy = ligy; M.bond_u = bond_u;
@@ -432,7 +435,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     A(c->t1, z.nt) A(c->t2, z.nt) A(c->t3, z.nt)
     A(c->l0, z.ne) A(c->kappap, z.ne) A(c->tarea, z.nt) A(c->tri_area, z.nt) A(c->wca, 3 * z.nv)
     A(c->slot, 3 * z.scr) A(c->inc_off, z.nv + n) A(c->inc_ent, z.inc)
-    A(c->upos, 4 * (int64_t)z.nv) A(c->part, 2 * (int64_t)z.part) A(c->tri_kin, 2 * (int64_t)IRBC_TRI_KIN * z.nt)
+    A(c->upos, 4 * (int64_t)z.nv) A(c->part, 2 * (int64_t)z.part) A(c->tri_kin, 2 * (int64_t)IRBC_TRI_KIN * z.nt) A(c->tri_sph, 2 * (int64_t)z.nt)
     A(c->ncand, z.nv) A(c->cand, (int64_t)z.nv * IRBC_MAX_CAND) A(c->draw_off, z.nv)
     A(c->flag, z.nv) A(c->list, z.nv)
     A(c->ligi, z.nlig) A(c->ligx, z.nligx) A(c->ligy, z.nligy)
@@ -1024,7 +1027,7 @@ static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t 
 }
 
 #define IRBC_GRAPH_STEPS 10 /* steps per captured chunk: one collision period */
-#define IRBC_TUNE_CANDIDATES 16 /* membrane shape x near shape x graphs x XCD chunking */
+#define IRBC_TUNE_CANDIDATES 32 /* membrane shape x near shape x graphs x XCD chunking x stream shape */
 
 /* grid for a kernel that maps blocks through xcd_tile */
 static unsigned xcd_grid(unsigned n) { return IRBC_NXCD * ((n + IRBC_NXCD - 1) / IRBC_NXCD); }
@@ -1079,25 +1082,52 @@ static unsigned near_blocks(const irbc_ctx *c)
 static int launch_near(irbc_ctx *c)
 {
     const double *tk = c->tri_kin + (c->par & 1) * c->tri_stride();
+    const float4 *ts = c->tri_sph + (c->par & 1) * c->z.nt;
     if (c->near_merged) {
         k_near<<<dim3(near_blocks(c), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(c->d_rep, c->lloff, c->llent,
-                                                                             c->nearq, tk, c->par);
+                                                                             c->nearq, tk, ts, c->par);
         LAUNCHCHK();
         return IRBC_OK;
     }
     k_near_plane<<<dim3(near_blocks(c), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
-        c->d_rep, c->lloff, c->llent, c->nearq, c->deferq, tk, c->par);
+        c->d_rep, c->lloff, c->llent, c->nearq, c->deferq, tk, ts, c->par);
     k_near_hit<<<dim3(std::max(16u, near_blocks(c) / 8u), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
         c->d_rep, c->lloff, c->llent, c->deferq, tk, c->par);
     LAUNCHCHK();
     return IRBC_OK;
 }
 
+/* the bulk streaming kernel in the chosen shape; e0/e1 (both or neither)
+ * are events bound to the dispatch */
+static int launch_bulk(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1)
+{
+    const unsigned T = (unsigned)c->z.tiles;
+    if (c->stream4) {
+        const unsigned G = (T + IRBC_WG / 64 - 1) / (IRBC_WG / 64);
+        if (e0)
+            hipExtLaunchKernelGGL(k_stream4, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep,
+                                  (const int32_t *)c->d_tile_rep, (int)T, (const int32_t *)c->skey2,
+                                  (const double *)c->ctab, (const int32_t *)c->lloff, c->nearq, c->par);
+        else
+            k_stream4<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, c->skey2, c->ctab, c->lloff,
+                                                c->nearq, c->par);
+    } else {
+        if (e0)
+            hipExtLaunchKernelGGL(k_stream, dim3(T), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep,
+                                  (const int32_t *)c->d_tile_rep, (const int32_t *)c->skey2,
+                                  (const double *)c->ctab, (const int32_t *)c->lloff, c->nearq, c->par);
+        else
+            k_stream<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab, c->lloff, c->nearq,
+                                               c->par);
+    }
+    LAUNCHCHK();
+    return IRBC_OK;
+}
+
 static int launch_stream(irbc_ctx *c)
 {
-    k_stream<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab,
-                                                         c->lloff, c->nearq, c->par);
-    LAUNCHCHK();
+    int rc = launch_bulk(c, nullptr, nullptr);
+    if (rc) return rc;
     return launch_near(c);
 }
 
@@ -1334,6 +1364,7 @@ int irbc_autotune(irbc_ctx *c, int32_t steps, double *report)
         c->near_merged = (k >> 1) & 1;
         c->use_graphs = (k >> 2) & 1;
         c->xcd_chunk = (k >> 3) & 1;
+        c->stream4 = (k >> 4) & 1;
         if (!rc) rc = snap_restore(c, S);
         if (!rc) rc = irbc_md_steps(c, steps); /* warm-up; also captures graphs */
         double ms = 0.0;
@@ -1365,6 +1396,7 @@ int irbc_autotune(irbc_ctx *c, int32_t steps, double *report)
     c->near_merged = (best_k >> 1) & 1;
     c->use_graphs = (best_k >> 2) & 1;
     c->xcd_chunk = (best_k >> 3) & 1;
+    c->stream4 = (best_k >> 4) & 1;
     return best_k;
 }
 
@@ -1608,16 +1640,14 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec)
         /* events bound to the dispatch itself: the kernel's own duration */
         e0 = c->event();
         e1 = c->event();
-        hipExtLaunchKernelGGL(k_stream, dim3((unsigned)c->z.tiles), dim3(IRBC_WG), 0, c->st, e0, e1, 0,
-                              c->d_rep, (const int32_t *)c->d_tile_rep,
-                              (const int32_t *)c->skey2, (const double *)c->ctab,
-                              (const int32_t *)c->lloff, c->nearq, c->par);
+        rc = launch_bulk(c, e0, e1);
+        if (rc) return rc;
         c->pend[0].push_back({e0, e1});
         e0 = c->event();
         hipEventRecord(e0, c->st);
     } else {
-        k_stream<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab,
-                                                             c->lloff, c->nearq, c->par);
+        rc = launch_bulk(c, nullptr, nullptr);
+        if (rc) return rc;
     }
     rc = launch_near(c);
     if (rc) return rc;
@@ -1646,7 +1676,8 @@ static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
 {
     const uint64_t key = flags | ((uint64_t)(c->par & 1) << 48) | ((uint64_t)n << 50) |
                          ((uint64_t)(c->mem_fused & 1) << 58) | ((uint64_t)(c->diag_on & 1) << 59) |
-                         ((uint64_t)(c->near_merged & 1) << 60) | ((uint64_t)(c->xcd_chunk & 1) << 61);
+                         ((uint64_t)(c->near_merged & 1) << 60) | ((uint64_t)(c->xcd_chunk & 1) << 61) |
+                         ((uint64_t)(c->stream4 & 1) << 62);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         const int par0 = c->par;
@@ -1734,6 +1765,9 @@ int irbc_set_option(irbc_ctx *c, int option, int value)
         return IRBC_OK;
     case IRBC_OPT_XCD_CHUNK:
         c->xcd_chunk = value != 0;
+        return IRBC_OK;
+    case IRBC_OPT_STREAM4:
+        c->stream4 = value != 0;
         return IRBC_OK;
     default:
         return fail(IRBC_ERR_ARG, "irbc_set_option: unknown option %d", option);

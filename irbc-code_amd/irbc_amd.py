@@ -33,6 +33,8 @@ OPT_MEMBRANE_FUSED = 1
 OPT_GRAPHS = 2
 OPT_NEAR_MERGED = 3
 OPT_XCD_CHUNK = 4
+OPT_STREAM4 = 5
+OPTIONS = (("membrane_fused", 1), ("near_merged", 3), ("graphs", 2), ("xcd_chunk", 4), ("stream4", 5))
 
 SERIES_FIELDS = ("step", "x", "y", "z", "vx", "vy", "vz", "bonds")
 
@@ -398,8 +400,9 @@ class Batch:
 
     def autotune(self, steps=20):
         """Pick the fastest launch options for this batch; returns
-        (chosen index, ms per step of each of the 16 candidates)."""
-        rep = np.zeros(16)
+        (chosen index, ms per step of each of the 32 candidates); bit i of
+        the index is option OPTIONS[i]."""
+        rep = np.zeros(32)
         k = _chk(self.L.irbc_autotune(self.h, int(steps), _ptr(rep, _d)))
         return k, rep
 

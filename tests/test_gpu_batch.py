@@ -131,12 +131,11 @@ def test_autotune_leaves_the_trajectory_alone():
     b1 = en.batch(ds)
     b1.md_steps(7)
     k, rep = b1.autotune(12)
-    assert 0 <= k < 16 and np.all(rep > 0)
+    assert 0 <= k < 32 and np.all(rep > 0)
     b1.md_steps(25)
     b1.sync()
     b2 = en.batch(ds)
-    for opt, bit in ((ia.OPT_MEMBRANE_FUSED, 0), (ia.OPT_NEAR_MERGED, 1), (ia.OPT_GRAPHS, 2),
-                     (ia.OPT_XCD_CHUNK, 3)):
+    for bit, (_, opt) in enumerate(ia.OPTIONS):
         b2.set_option(opt, (k >> bit) & 1)
     b2.md_steps(32)
     b2.sync()
@@ -148,13 +147,15 @@ def test_autotune_leaves_the_trajectory_alone():
 
 @pytest.mark.gpu
 def test_launch_options_do_not_change_results():
-    """Near-membrane shape and XCD chunking are pure launch choices."""
+    """Near-membrane shape, XCD chunking and the stream shape are pure
+    launch choices."""
     ds = _mix()
     snaps = []
-    for near, xcd in ((1, 1), (0, 1), (1, 0)):
+    for near, xcd, s4 in ((1, 1, 0), (0, 1, 0), (1, 0, 0), (1, 1, 1)):
         b = en.batch(ds)
         b.set_option(ia.OPT_NEAR_MERGED, near)
         b.set_option(ia.OPT_XCD_CHUNK, xcd)
+        b.set_option(ia.OPT_STREAM4, s4)
         b.md_steps(31)
         b.sync()
         snaps.append([_snapshot(b, r) for r in range(len(ds))])
