@@ -197,7 +197,7 @@ struct irbc_ctx {
     int par = 0;                /* parity of the current step: picks the double-buffered state */
     int mem_fused = 0;          /* membrane step as one workgroup per replica (1) or five grid kernels (0) */
     int use_graphs = 1;         /* irbc_md_steps replays captured chunks of IRBC_GRAPH_STEPS steps */
-    int near_merged = 1;        /* near-membrane streaming in one kernel (1) or plane pass + hit pass (0) */
+    int near_merged = 0;        /* near-membrane streaming in one kernel (1) or sphere pass + hit pass (0) */
     int xcd_chunk = 1;          /* sort and cell kernels hand each XCD a contiguous eighth of the tiles */
     int stream4 = 0;            /* bulk streaming with four particles per lane */
     std::map<uint64_t, hipGraphExec_t> graphs;
@@ -1089,10 +1089,10 @@ static int launch_near(irbc_ctx *c)
         LAUNCHCHK();
         return IRBC_OK;
     }
-    k_near_plane<<<dim3(near_blocks(c), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
-        c->d_rep, c->lloff, c->llent, c->nearq, c->deferq, tk, ts, c->par);
+    k_near_sphere<<<dim3(std::max(4u, near_blocks(c) / 8u), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
+        c->d_rep, c->lloff, c->llent, c->nearq, c->deferq, ts, c->par);
     k_near_hit<<<dim3(std::max(16u, near_blocks(c) / 8u), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
-        c->d_rep, c->lloff, c->llent, c->deferq, tk, c->par);
+        c->d_rep, c->lloff, c->llent, c->deferq, tk, ts, c->par);
     LAUNCHCHK();
     return IRBC_OK;
 }
