@@ -174,7 +174,7 @@ int64_t bytes_of(const Sizes &z, int n)
 {
     int64_t b = 0;
     b += z.slots * (2 * 6 * 4 + 2 * 4 + 4 * 4);           /* particles x2, ids x2, keys, ranks, sorted keys, queue */
-    b += z.cells * (4 + 8 * 8 + 4 + 4 + 8 * 8 + 4);       /* count, ctab, vhead, llcount, diag, cursor */
+    b += z.cells * (4 + 8 * 8 + 4 + 4 + 8 * 8 + 4 + 4);   /* count, ctab, vhead, llcount, diag, cursor, reach */
     b += z.cells * 8 * 8;                                 /* tapes */
     b += z.cells1 * (4 + 4);                              /* start, lloff */
     b += z.llent * 4;
@@ -220,6 +220,7 @@ struct irbc_ctx {
     /* cells */
     int32_t *ccount, *cstart, *vhead, *lloff, *llcount, *llent;
     double *ctab;
+    float *cdmin;
     CellDiag diag;
     double *cell_u, *wall_z, *mem_z;
     /* membrane */
@@ -423,7 +424,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     A(c->ida, z.slots) A(c->idb, z.slots) A(c->skey, z.slots) A(c->srank, z.slots) A(c->skey2, z.slots) A(c->nearq, z.slots)
     A(c->deferq, 2 * z.slots)
     A(c->ccount, z.cells) A(c->cstart, z.cells1) A(c->vhead, z.cells) A(c->lloff, z.cells1)
-    A(c->llcount, z.cells) A(c->llent, z.llent) A(c->ctab, z.cells * 8)
+    A(c->llcount, z.cells) A(c->llent, z.llent) A(c->ctab, z.cells * 8) A(c->cdmin, z.cells)
     A(c->diag.vcm, 3 * z.cells) A(c->diag.ncc, z.cells) A(c->diag.rand1, z.cells)
     A(c->diag.rand2, z.cells) A(c->diag.dvv, z.cells) A(c->diag.the, z.cells) A(c->diag.nc, z.cells)
     A(c->cell_u, 2 * z.cells) A(c->wall_z, 3 * z.cells) A(c->mem_z, 3 * z.cells)
@@ -1090,7 +1091,7 @@ static int launch_near(irbc_ctx *c)
         return IRBC_OK;
     }
     k_near_sphere<<<dim3(std::max(4u, near_blocks(c) / 8u), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
-        c->d_rep, c->lloff, c->llent, c->nearq, c->deferq, ts, c->par);
+        c->d_rep, c->lloff, c->llent, c->nearq, c->deferq, tk, ts, c->par);
     k_near_hit<<<dim3(std::max(16u, near_blocks(c) / 8u), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
         c->d_rep, c->lloff, c->llent, c->deferq, tk, ts, c->par);
     LAUNCHCHK();
@@ -1102,23 +1103,27 @@ static int launch_near(irbc_ctx *c)
 static int launch_bulk(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1)
 {
     const unsigned T = (unsigned)c->z.tiles;
+    k_cell_reach<<<(unsigned)c->z.ctiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, c->lloff, c->llent,
+                                                              c->tri_sph + (c->par & 1) * c->z.nt, c->cdmin);
     if (c->stream4) {
         const unsigned G = (T + IRBC_WG / 64 - 1) / (IRBC_WG / 64);
         if (e0)
             hipExtLaunchKernelGGL(k_stream4, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep,
                                   (const int32_t *)c->d_tile_rep, (int)T, (const int32_t *)c->skey2,
-                                  (const double *)c->ctab, (const int32_t *)c->lloff, c->nearq, c->par);
+                                  (const double *)c->ctab, (const int32_t *)c->lloff, (const float *)c->cdmin,
+                                  c->nearq, c->par);
         else
             k_stream4<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, c->skey2, c->ctab, c->lloff,
-                                                c->nearq, c->par);
+                                                c->cdmin, c->nearq, c->par);
     } else {
         if (e0)
             hipExtLaunchKernelGGL(k_stream, dim3(T), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep,
                                   (const int32_t *)c->d_tile_rep, (const int32_t *)c->skey2,
-                                  (const double *)c->ctab, (const int32_t *)c->lloff, c->nearq, c->par);
+                                  (const double *)c->ctab, (const int32_t *)c->lloff, (const float *)c->cdmin,
+                                  c->nearq, c->par);
         else
-            k_stream<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab, c->lloff, c->nearq,
-                                               c->par);
+            k_stream<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab, c->lloff, c->cdmin,
+                                               c->nearq, c->par);
     }
     LAUNCHCHK();
     return IRBC_OK;
@@ -1152,18 +1157,15 @@ static int launch_membrane(irbc_ctx *c, int phases, hipStream_t s)
     }
     const unsigned X = IRBC_NXCD;
     const dim3 gv(X * (unsigned)((c->z.maxnv + IRBC_WG - 1) / IRBC_WG), nr);
-    const dim3 ge(X * (unsigned)((std::max(c->z.maxne, c->z.maxnt) + IRBC_WG - 1) / IRBC_WG), nr);
+    /* element blocks, then vertex blocks for the bond preparation */
+    const dim3 ge(X * (unsigned)((std::max(c->z.maxne, c->z.maxnt) + IRBC_WG - 1) / IRBC_WG +
+                                 (c->z.maxnv + IRBC_WG - 1) / IRBC_WG), nr);
     const dim3 gt(X * (unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), nr);
     k_mem_begin<<<gv, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
-    if (phases & (PH_VA | PH_FORCES)) {
-        k_mem_edges<<<ge, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
-        k_mem_tris<<<gt, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
-    }
-    if (phases & (PH_FORCES | PH_EXTERNAL | PH_BONDS | PH_KICK2))
-        k_mem_gather<<<gv, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
-    if (phases & PH_KICK2)
-        k_tri_kin<<<dim3((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), nr), IRBC_WG, 0, s>>>(c->d_rep, M);
-    k_mem_finish<<<nr, IRBC_MEM_WG, 0, s>>>(c->d_rep, phases, M, c->par);
+    if (phases & (PH_VA | PH_FORCES | PH_BONDS)) k_mem_edges<<<ge, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
+    if (phases & (PH_VA | PH_FORCES)) k_mem_tris<<<gt, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
+    if (phases & (PH_FORCES | PH_EXTERNAL | PH_KICK2)) k_mem_gather<<<gv, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
+    k_mem_finish<<<nr, IRBC_MEM_WG, 0, s>>>(c->d_rep, phases, M, c->par); /* also the next triangle kinematics */
     LAUNCHCHK();
     return IRBC_OK;
 }
