@@ -26,6 +26,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "irbc-code_amd"))
 
@@ -64,9 +66,54 @@ def cpu_baseline(b, d, st, budget_s=15.0, max_steps=5000):
         n += 1
     dt = time.perf_counter() - t0
     ns = len(s["x"])
+    del o
     return {"value": ns * n / dt, "unit": "particle-steps/s", "cores": int(ncores), "kind": "port",
             "sample": "FP64 oracle, same 64x64x48 state, %d MD steps (%d collisions), %.1f s"
-                      % (n, n // 10, dt)}
+                      % (n, n // 10, dt),
+            "accuracy": accuracy(b, pp, st, po)}
+
+
+def accuracy(b, pp, st, po):
+    """The oracle as a checker on the benchmark's own state: one SRD
+    collision with the oracle's draws injected, then the membrane's area,
+    volume and forces; relative errors of the GPU path (FP32-stored solvent,
+    exact fixed-point cell sums, FP64 membrane) against the FP64 oracle."""
+    o = po.Oracle(pp)
+    o.set_mesh(st["V"], st["E"], st["T"])
+    s = b.solvent(0)
+    o.set_solvent(s["x"], s["y"], s["z"], s["vx"], s["vy"], s["vz"], s["indio"])
+    v = b.vertices(0)
+    o.set_vertices(**{k: v[k] for k in po.Oracle.VD + po.Oracle.VI})
+    o.set_parasite(*b.parasite(0))
+    o.set_ligands(b.ligands(0))
+    w = b.globals(0)["wallspeed"]
+    o.set_wallspeed(w)
+    o.seed_mt(11, 22)
+    o.collision()
+    b.set_diagnostics(True)
+    b.set_collision_tape(0, *o.get_collision_tape())
+    b.collision_step()
+    co, cg = o.cells(), b.cells(0)
+    po_ = co["vcm"].reshape(-1, 3) * co["ncc"][:, None]
+    pg_ = cg["vcm"].reshape(-1, 3) * cg["ncc"][:, None]
+    cell_p = float(np.abs(pg_ - po_).max() / np.abs(po_).max())
+    so, sg = o.solvent(), b.solvent(0)
+    vel = max(float(np.abs(sg[k] - so[k]).max() / np.abs(so[k]).max()) for k in ("vx", "vy", "vz"))
+    keys_equal = bool(np.array_equal(sg["key"], so["pid"]) and np.array_equal(cg["nc"], co["nc"]))
+    o.totalva()
+    o.pfdt()
+    o.forces()
+    o.external()
+    b.membrane_forces()
+    go, gg = o.globals(), b.globals(0)
+    vo, vg = o.vertices(), b.vertices(0)
+    force = max(float(np.abs(vg[k] - vo[k]).max() / np.abs(vo[k]).max()) for k in ("fx", "fy", "fz"))
+    return {"cell_momentum_rel": cell_p, "rotated_velocity_rel_fp32_storage": vel,
+            "cell_keys_and_counts_bit_exact": keys_equal,
+            "area_rel": abs(gg["Sarea"] - go["Sarea"]) / go["Sarea"],
+            "volume_rel": abs(gg["Volume"] - go["Volume"]) / go["Volume"],
+            "membrane_force_rel": force,
+            "targets": {"cell_momentum_rel": 1e-5, "area_volume_rel": 1e-8}}
 
 
 def load_traffic(name):
@@ -153,6 +200,8 @@ def main():
     ap.add_argument("--served-rate", type=float, default=4.0)
     ap.add_argument("--served-steps", type=int, default=5000)
     ap.add_argument("--profile-every", type=int, default=4)
+    ap.add_argument("--options", type=int, default=-1,
+                    help="launch-option index (bit i = irbc_amd.OPTIONS[i]) instead of autotuning")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the replicas of this workload are independent "
@@ -165,7 +214,12 @@ def main():
     st = en.load(b, 0, d)
     en.start(b)
     ns = en.n_solvent(d)
-    tuned, trials = b.autotune(50)  # state is restored: the trajectory does not advance
+    if a.options >= 0:
+        tuned, trials = a.options, np.zeros(0)
+        for i, (_, opt) in enumerate(ia.OPTIONS):
+            b.set_option(opt, (tuned >> i) & 1)
+    else:
+        tuned, trials = b.autotune(50)  # state is restored: the trajectory does not advance
     b.md_steps(a.warmup)
     b.sync()
     t0 = time.perf_counter()

@@ -1144,9 +1144,14 @@ static int launch_tri_kin(irbc_ctx *c)
     return IRBC_OK;
 }
 
-/* the membrane part of one step on stream `s`: the next step's triangle
- * kinematics go to the other parity's buffer */
-static int launch_membrane(irbc_ctx *c, int phases, hipStream_t s)
+/* The membrane part of one step on stream `s`, as a list of launches so
+ * that the time loop can interleave them with the solvent chain's on the
+ * host (both chains then start at once).  The next step's triangle
+ * kinematics go to the other parity's buffer.  Returns the number of
+ * launches; launch k of them with membrane_launch(c, phases, s, k). */
+static int membrane_launches(const irbc_ctx *c) { return c->mem_fused ? 1 : 6; }
+
+static int membrane_launch(irbc_ctx *c, int phases, hipStream_t s, int k)
 {
     const MemDev M = c->mem(c->par ^ 1);
     const unsigned nr = (unsigned)c->nrep;
@@ -1161,12 +1166,33 @@ static int launch_membrane(irbc_ctx *c, int phases, hipStream_t s)
     const dim3 ge(X * (unsigned)((std::max(c->z.maxne, c->z.maxnt) + IRBC_WG - 1) / IRBC_WG +
                                  (c->z.maxnv + IRBC_WG - 1) / IRBC_WG), nr);
     const dim3 gt(X * (unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), nr);
-    k_mem_begin<<<gv, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
-    if (phases & (PH_VA | PH_FORCES | PH_BONDS)) k_mem_edges<<<ge, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
-    if (phases & (PH_VA | PH_FORCES)) k_mem_tris<<<gt, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
-    if (phases & (PH_FORCES | PH_EXTERNAL | PH_KICK2)) k_mem_gather<<<gv, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
-    k_mem_finish<<<nr, IRBC_MEM_WG, 0, s>>>(c->d_rep, phases, M, c->par); /* also the next triangle kinematics */
+    switch (k) {
+    case 0: k_mem_begin<<<gv, IRBC_WG, 0, s>>>(c->d_rep, phases, M); break;
+    case 1:
+        if (phases & (PH_VA | PH_FORCES | PH_BONDS)) k_mem_edges<<<ge, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
+        break;
+    case 2:
+        if (phases & (PH_VA | PH_FORCES)) k_mem_tris<<<gt, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
+        break;
+    case 3:
+        if (phases & (PH_FORCES | PH_EXTERNAL | PH_KICK2)) k_mem_gather<<<gv, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
+        break;
+    case 4:
+        if (phases & PH_KICK2)
+            k_tri_kin<<<dim3((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), nr), IRBC_WG, 0, s>>>(c->d_rep, M);
+        break;
+    default: k_mem_finish<<<nr, IRBC_FIN_WG, 0, s>>>(c->d_rep, phases, M, c->par); break;
+    }
     LAUNCHCHK();
+    return IRBC_OK;
+}
+
+static int launch_membrane(irbc_ctx *c, int phases, hipStream_t s)
+{
+    for (int k = 0; k < membrane_launches(c); ++k) {
+        const int rc = membrane_launch(c, phases, s, k);
+        if (rc) return rc;
+    }
     return IRBC_OK;
 }
 
@@ -1633,11 +1659,14 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec)
     }
     HIPCHK(hipEventRecord(c->ev_pre, c->st));
     HIPCHK(hipStreamWaitEvent(c->st2, c->ev_pre, 0));
-    if (rec) { e0 = c->event(); hipEventRecord(e0, c->st2); }
-    rc = launch_membrane(c, mem_phases, c->st2);
+    /* the two chains' launches alternate on the host, membrane first: the
+     * host's launch latency then delays neither chain's start */
+    hipEvent_t m0 = nullptr, s0 = nullptr;
+    if (rec) { m0 = c->event(); hipEventRecord(m0, c->st2); }
+    const int nm = membrane_launches(c);
+    int km = 0;
+    rc = membrane_launch(c, mem_phases, c->st2, km++);
     if (rc) return rc;
-    if (rec) { e1 = c->event(); hipEventRecord(e1, c->st2); c->pend[2].push_back({e0, e1}); }
-    HIPCHK(hipEventRecord(c->ev_mem, c->st2));
     if (rec) {
         /* events bound to the dispatch itself: the kernel's own duration */
         e0 = c->event();
@@ -1645,15 +1674,25 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec)
         rc = launch_bulk(c, e0, e1);
         if (rc) return rc;
         c->pend[0].push_back({e0, e1});
-        e0 = c->event();
-        hipEventRecord(e0, c->st);
+        s0 = c->event();
+        hipEventRecord(s0, c->st);
     } else {
         rc = launch_bulk(c, nullptr, nullptr);
         if (rc) return rc;
     }
+    if (km < nm) {
+        rc = membrane_launch(c, mem_phases, c->st2, km++);
+        if (rc) return rc;
+    }
     rc = launch_near(c);
     if (rc) return rc;
-    if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[3].push_back({e0, e1}); }
+    if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[3].push_back({s0, e1}); }
+    while (km < nm) {
+        rc = membrane_launch(c, mem_phases, c->st2, km++);
+        if (rc) return rc;
+    }
+    if (rec) { e1 = c->event(); hipEventRecord(e1, c->st2); c->pend[2].push_back({m0, e1}); }
+    HIPCHK(hipEventRecord(c->ev_mem, c->st2));
     HIPCHK(hipStreamWaitEvent(c->st, c->ev_mem, 0));
     LAUNCHCHK();
     return IRBC_OK;
