@@ -1073,6 +1073,17 @@ static int launch_llist(irbc_ctx *c, int staged)
     return IRBC_OK;
 }
 
+/* the cell reach distances from the current lists and triangle spheres;
+ * staged: every replica, else those that collide or make lists this step */
+static int launch_cell_reach(irbc_ctx *c, int staged)
+{
+    k_cell_reach<<<(unsigned)c->z.ctiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, staged, c->par & 1,
+                                                              c->lloff, c->llent,
+                                                              c->tri_sph + (c->par & 1) * c->z.nt, c->cdmin);
+    LAUNCHCHK();
+    return IRBC_OK;
+}
+
 static unsigned near_blocks(const irbc_ctx *c)
 {
     int64_t maxnp = 0;
@@ -1103,8 +1114,6 @@ static int launch_near(irbc_ctx *c)
 static int launch_bulk(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1)
 {
     const unsigned T = (unsigned)c->z.tiles;
-    k_cell_reach<<<(unsigned)c->z.ctiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, c->lloff, c->llent,
-                                                              c->tri_sph + (c->par & 1) * c->z.nt, c->cdmin);
     if (c->stream4) {
         const unsigned G = (T + IRBC_WG / 64 - 1) / (IRBC_WG / 64);
         if (e0)
@@ -1139,7 +1148,7 @@ static int launch_stream(irbc_ctx *c)
 static int launch_tri_kin(irbc_ctx *c)
 {
     k_tri_kin<<<dim3((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), (unsigned)c->nrep), IRBC_WG, 0,
-                c->st>>>(c->d_rep, c->mem(c->par));
+                c->st>>>(c->d_rep, c->mem(c->par), 0);
     LAUNCHCHK();
     return IRBC_OK;
 }
@@ -1179,7 +1188,7 @@ static int membrane_launch(irbc_ctx *c, int phases, hipStream_t s, int k)
         break;
     case 4:
         if (phases & PH_KICK2)
-            k_tri_kin<<<dim3((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), nr), IRBC_WG, 0, s>>>(c->d_rep, M);
+            k_tri_kin<<<dim3((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), nr), IRBC_WG, 0, s>>>(c->d_rep, M, 1);
         break;
     default: k_mem_finish<<<nr, IRBC_FIN_WG, 0, s>>>(c->d_rep, phases, M, c->par); break;
     }
@@ -1232,6 +1241,8 @@ int irbc_stream(irbc_ctx *c)
     int rc = ready(c, true);
     if (rc) return rc;
     int rc2 = launch_tri_kin(c);
+    if (rc2) return rc2;
+    rc2 = launch_cell_reach(c, 1);
     if (rc2) return rc2;
     rc2 = launch_stream(c);
     if (rc2) return rc2;
@@ -1657,6 +1668,10 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec)
         rc = launch_llist(c, 0);
         if (rc) return rc;
     }
+    if (col || ll) {
+        rc = launch_cell_reach(c, 0); /* replicas that collided or made lists: keeps the drift small */
+        if (rc) return rc;
+    }
     HIPCHK(hipEventRecord(c->ev_pre, c->st));
     HIPCHK(hipStreamWaitEvent(c->st2, c->ev_pre, 0));
     /* the two chains' launches alternate on the host, membrane first: the
@@ -1759,6 +1774,8 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
     if (nsteps == 0) return IRBC_OK;
     k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 1);
     rc = launch_tri_kin(c);
+    if (rc) return rc;
+    rc = launch_cell_reach(c, 1);
     if (rc) return rc;
     if (c->use_graphs && !c->prof) {
         const int CH = IRBC_GRAPH_STEPS;

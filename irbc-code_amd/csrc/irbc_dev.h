@@ -40,6 +40,9 @@
 
 #define IRBC_SERIES_REC 8      /* doubles per time-series record */
 
+/* vertex drift up to which the cell reach distances stand (see k_cell_reach) */
+#define IRBC_DRIFT_MAX 0.25
+
 struct DevRep {
     /* ---- geometry and packing ---- */
     int32_t Lx, Ly, Lz, ncell;
@@ -86,6 +89,13 @@ struct DevRep {
     int32_t near_count[2];  /* particles queued for bounce-back, by step parity */
     int32_t defer_count[2]; /* of those, particles that cross a triangle plane */
     int32_t wca_count;      /* vertices inside the parasite's cutoff this step */
+    /* cell reach bookkeeping: float bits of the largest vertex speed after
+     * the first half-kick (a vertex moves that speed x dt in the step) of
+     * this step and of the coming one; by step parity, the sum of those
+     * moves over the steps since the reach distances were made, before this
+     * step, and the margin this step's streaming subtracts from them */
+    uint32_t vcur_bits, vnext_bits;
+    double reach_acc[2], reach_drift[2];
     int32_t draw_total;     /* bond draws needed this step */
     /* time series: every `series_every` steps one record of IRBC_SERIES_REC
      * doubles (step, centroid x y z unwrapped, centroid velocity, bonds) */
