@@ -178,7 +178,7 @@ int64_t bytes_of(const Sizes &z, int n)
     b += z.cells * 8 * 8;                                 /* tapes */
     b += z.cells1 * (4 + 4);                              /* start, lloff */
     b += z.llent * 4;
-    b += (int64_t)z.nv * (19 * 8 + 12 * 4 + IRBC_MAX_CAND * (4 + 8));
+    b += (int64_t)z.nv * (31 * 8 + 12 * 4 + IRBC_MAX_CAND * (4 + 8)); /* incl. vertex kinematics x2 */
     b += 2 * (int64_t)z.nt * (IRBC_TRI_KIN * 8 + 16) + (int64_t)z.part * 16 + z.slots * 8;
     b += (int64_t)z.ne * (5 * 4 + 2 * 8) + (int64_t)z.nt * (3 * 4 + 2 * 8);
     b += z.scr * 24 + (int64_t)z.inc * 4 + (int64_t)(z.nv + n) * 4;
@@ -196,6 +196,7 @@ struct irbc_ctx {
     hipEvent_t ev_pre = nullptr, ev_mem = nullptr;
     int par = 0;                /* parity of the current step: picks the double-buffered state */
     int mem_fused = 0;          /* membrane step as one workgroup per replica (1) or five grid kernels (0) */
+    int mem_lds_nv = 0;         /* largest mesh whose unwrapped positions the one-workgroup step keeps in LDS */
     int use_graphs = 1;         /* irbc_md_steps replays captured chunks of IRBC_GRAPH_STEPS steps */
     int near_merged = 0;        /* near-membrane streaming in one kernel (1) or sphere pass + hit pass (0) */
     int xcd_chunk = 1;          /* sort and cell kernels hand each XCD a contiguous eighth of the tiles */
@@ -227,7 +228,7 @@ struct irbc_ctx {
     double *mx, *my, *mz, *mvx, *mvy, *mvz, *mfx, *mfy, *mfz, *poff, *pon;
     int32_t *mix, *miy, *mpid, *bondi, *bondl, *knob, *vnext;
     int32_t *e1, *e2, *es1, *es2, *iedge, *t1, *t2, *t3;
-    double *l0, *kappap, *tarea, *tri_area, *wca, *slot, *upos, *part, *tri_kin;
+    double *l0, *kappap, *tarea, *tri_area, *wca, *slot, *upos, *part, *tri_kin, *vkin;
     float4 *tri_sph;
     int32_t *inc_off, *inc_ent;
     int32_t *ncand, *cand, *draw_off, *flag, *list;
@@ -274,8 +275,9 @@ struct irbc_ctx {
         M.t1 = t1; M.t2 = t2; M.t3 = t3;
         M.l0 = l0; M.kappap = kappap; M.tarea = tarea;
         M.tri_area = tri_area; M.wca = wca; M.slot = slot;
-        M.upos = upos; M.part = part; M.tri_kin = tri_kin + (tbuf & 1) * tri_stride();
+        M.upos = upos; M.upos_base = 0; M.part = part; M.tri_kin = tri_kin + (tbuf & 1) * tri_stride();
         M.tri_sph = tri_sph + (tbuf & 1) * z.nt;
+        M.vkin = vkin + (tbuf & 1) * 6 * (int64_t)z.nv;
         M.inc_off = inc_off; M.inc_ent = inc_ent;
         M.ncand = ncand; M.cand = cand; M.draw_off = draw_off; M.flag = flag; M.list = list;
         M.ligi = ligi; M.ligx = ligx; M.ligy = ligy; M.bond_u = bond_u;
@@ -402,6 +404,13 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     irbc_ctx *c = new irbc_ctx();
     c->device = device;
     c->nrep = n;
+    {
+        /* dynamic LDS of the one-workgroup membrane step: 32 bytes a vertex,
+         * within 64 KiB per workgroup less 16 KiB for its static arrays */
+        int lim = 65536;
+        if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lim = 65536;
+        c->mem_lds_nv = std::max(0, (std::min(lim, 65536) - 16384) / 32);
+    }
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_pre, hipEventDisableTiming) != hipSuccess ||
@@ -436,7 +445,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     A(c->t1, z.nt) A(c->t2, z.nt) A(c->t3, z.nt)
     A(c->l0, z.ne) A(c->kappap, z.ne) A(c->tarea, z.nt) A(c->tri_area, z.nt) A(c->wca, 3 * z.nv)
     A(c->slot, 3 * z.scr) A(c->inc_off, z.nv + n) A(c->inc_ent, z.inc)
-    A(c->upos, 4 * (int64_t)z.nv) A(c->part, 2 * (int64_t)z.part) A(c->tri_kin, 2 * (int64_t)IRBC_TRI_KIN * z.nt) A(c->tri_sph, 2 * (int64_t)z.nt)
+    A(c->upos, 4 * (int64_t)z.nv) A(c->part, 2 * (int64_t)z.part) A(c->tri_kin, 2 * (int64_t)IRBC_TRI_KIN * z.nt) A(c->tri_sph, 2 * (int64_t)z.nt) A(c->vkin, 12 * (int64_t)z.nv)
     A(c->ncand, z.nv) A(c->cand, (int64_t)z.nv * IRBC_MAX_CAND) A(c->draw_off, z.nv)
     A(c->flag, z.nv) A(c->list, z.nv)
     A(c->ligi, z.nlig) A(c->ligx, z.nligx) A(c->ligy, z.nligy)
@@ -1145,27 +1154,36 @@ static int launch_stream(irbc_ctx *c)
     return launch_near(c);
 }
 
-static int launch_tri_kin(irbc_ctx *c)
+/* this step's triangle kinematics and bounding spheres from its vertex
+ * kinematics; refresh: first the vertex kinematics from the vertex arrays
+ * (else the previous step's membrane gather or this step's collision wrote
+ * them) */
+static int launch_tri_kin(irbc_ctx *c, int refresh)
 {
-    k_tri_kin<<<dim3((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), (unsigned)c->nrep), IRBC_WG, 0,
-                c->st>>>(c->d_rep, c->mem(c->par), 0);
+    const MemDev M = c->mem(c->par);
+    if (refresh)
+        k_vkin<<<dim3((unsigned)((c->z.maxnv + IRBC_WG - 1) / IRBC_WG), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
+            c->d_rep, M);
+    k_tri_kin<<<dim3((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
+        c->d_rep, M);
     LAUNCHCHK();
     return IRBC_OK;
 }
 
 /* The membrane part of one step on stream `s`, as a list of launches so
  * that the time loop can interleave them with the solvent chain's on the
- * host (both chains then start at once).  The next step's triangle
+ * host (both chains then start at once).  The next step's vertex
  * kinematics go to the other parity's buffer.  Returns the number of
  * launches; launch k of them with membrane_launch(c, phases, s, k). */
-static int membrane_launches(const irbc_ctx *c) { return c->mem_fused ? 1 : 6; }
+static int membrane_launches(const irbc_ctx *c) { return c->mem_fused ? 1 : 5; }
 
 static int membrane_launch(irbc_ctx *c, int phases, hipStream_t s, int k)
 {
     const MemDev M = c->mem(c->par ^ 1);
     const unsigned nr = (unsigned)c->nrep;
     if (c->mem_fused) {
-        k_mem_step<<<nr, IRBC_MEM_WG, 0, s>>>(c->d_rep, phases, M, c->par);
+        const int lds_nv = c->z.maxnv <= c->mem_lds_nv ? c->z.maxnv : 0;
+        k_mem_step<<<nr, IRBC_MEM_WG, (size_t)32 * lds_nv, s>>>(c->d_rep, phases, M, c->par, lds_nv);
         LAUNCHCHK();
         return IRBC_OK;
     }
@@ -1185,10 +1203,6 @@ static int membrane_launch(irbc_ctx *c, int phases, hipStream_t s, int k)
         break;
     case 3:
         if (phases & (PH_FORCES | PH_EXTERNAL | PH_KICK2)) k_mem_gather<<<gv, IRBC_WG, 0, s>>>(c->d_rep, phases, M);
-        break;
-    case 4:
-        if (phases & PH_KICK2)
-            k_tri_kin<<<dim3((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), nr), IRBC_WG, 0, s>>>(c->d_rep, M, 1);
         break;
     default: k_mem_finish<<<nr, IRBC_FIN_WG, 0, s>>>(c->d_rep, phases, M, c->par); break;
     }
@@ -1240,7 +1254,7 @@ int irbc_stream(irbc_ctx *c)
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     int rc = ready(c, true);
     if (rc) return rc;
-    int rc2 = launch_tri_kin(c);
+    int rc2 = launch_tri_kin(c, 1);
     if (rc2) return rc2;
     rc2 = launch_cell_reach(c, 1);
     if (rc2) return rc2;
@@ -1668,11 +1682,22 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec)
         rc = launch_llist(c, 0);
         if (rc) return rc;
     }
+    /* this step's triangle kinematics (from the vertex kinematics the
+     * previous membrane gather or this collision wrote; the membrane chain
+     * writes the other parity's) and, for replicas that collided or made
+     * lists, the cell reach distances, which the membrane finish's drift
+     * margin builds on: then both before the membrane chain starts */
     if (col || ll) {
-        rc = launch_cell_reach(c, 0); /* replicas that collided or made lists: keeps the drift small */
+        rc = launch_tri_kin(c, 0);
+        if (rc) return rc;
+        rc = launch_cell_reach(c, 0);
         if (rc) return rc;
     }
     HIPCHK(hipEventRecord(c->ev_pre, c->st));
+    if (!col && !ll) {
+        rc = launch_tri_kin(c, 0);
+        if (rc) return rc;
+    }
     HIPCHK(hipStreamWaitEvent(c->st2, c->ev_pre, 0));
     /* the two chains' launches alternate on the host, membrane first: the
      * host's launch latency then delays neither chain's start */
@@ -1773,7 +1798,7 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
     if (rc) return rc;
     if (nsteps == 0) return IRBC_OK;
     k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 1);
-    rc = launch_tri_kin(c);
+    rc = launch_tri_kin(c, 1);
     if (rc) return rc;
     rc = launch_cell_reach(c, 1);
     if (rc) return rc;

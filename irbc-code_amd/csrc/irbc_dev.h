@@ -17,7 +17,7 @@
 #include <stdint.h>
 
 #define IRBC_WG 256            /* particle and cell tile */
-#define IRBC_MEM_WG 1024       /* one membrane workgroup per replica */
+#define IRBC_MEM_WG 512        /* one membrane workgroup per replica (2 waves per SIMD: no spills) */
 #define IRBC_SCAN_BLOCK 4096   /* elements per scan block (256 threads x 16) */
 #define IRBC_MAX_CAND 16       /* ligand candidates per knob (3x3 window in practice) */
 #define IRBC_LLIST_PER_TRI 64  /* a triangle lies within 2a of at most 4x4x4 cells */

@@ -100,11 +100,7 @@ def test_graph_replay_equals_eager():
             np.testing.assert_array_equal(s1[k], s2[k], err_msg="replica %d %s" % (r, k))
 
 
-@pytest.mark.gpu
-def test_fused_membrane_matches_grid_kernels():
-    """The one-workgroup membrane step and the five grid-wide kernels differ
-    only in the summation order of the global area and volume."""
-    ds = _mix()
+def _fused_vs_grid(ds):
     out = []
     for fused in (0, 1):
         b = en.batch(ds)
@@ -121,6 +117,23 @@ def test_fused_membrane_matches_grid_kernels():
             np.testing.assert_array_equal(v1[k], v0[k])
         assert abs(g1["Sarea"] - g0["Sarea"]) <= 1e-12 * g0["Sarea"]
         assert abs(g1["Volume"] - g0["Volume"]) <= 1e-12 * g0["Volume"]
+
+
+@pytest.mark.gpu
+def test_fused_membrane_matches_grid_kernels():
+    """The one-workgroup membrane step and the five grid-wide kernels differ
+    only in the summation order of the global area and volume."""
+    _fused_vs_grid(_mix())
+
+
+@pytest.mark.gpu
+def test_fused_membrane_with_lds_positions_far_into_the_batch():
+    """Replicas whose vertices start far into the packed arrays (here 3 x 642
+    before the last), so the one-workgroup step's LDS copy of the unwrapped
+    positions is indexed relative to the replica, not by global index."""
+    ds = [en.replica(16, 16, 16, 6, 642, None, 1.0, 100, kon=0.5, koff=0.05, shear=0.05, seed=40 + i)
+          for i in range(4)]
+    _fused_vs_grid(ds)
 
 
 @pytest.mark.gpu
