@@ -168,14 +168,20 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
  *   lane (16-byte loads and stores); 0 (default): one per lane.  Same
  *   results. */
 #define IRBC_OPT_STREAM4 5
+/* IRBC_OPT_STREAM_GRID  1: bulk streaming on a grid of 4 workgroups per CU,
+ *   each looping over tiles, which leaves room on every CU for the membrane
+ *   kernels running beside it; 0 (default): one workgroup per tile.  Same
+ *   results. */
+#define IRBC_OPT_STREAM_GRID 6
 int irbc_set_option(irbc_ctx *ctx, int option, int value);
-/* Times `steps` MD steps under each of the 32 combinations of the options
- * above (membrane shape, near shape, graphs, XCD chunking, stream shape) on this
- * context's own replicas, then keeps the fastest.  The state is snapshot on
- * the device first and restored afterwards, so the trajectory is not
- * advanced.  report (may be NULL) receives 32 ms-per-step figures, index =
- * fused + 2 near_merged + 4 graphs + 8 xcd_chunk + 16 stream4.  Returns the chosen
- * index or < 0.  Needs free device memory for one copy of the context. */
+/* Times `steps` MD steps under each of the 64 combinations of the options
+ * above (membrane shape, near shape, graphs, XCD chunking, stream shape,
+ * stream grid) on this context's own replicas, then keeps the fastest.  The
+ * state is snapshot on the device first and restored afterwards, so the
+ * trajectory is not advanced.  report (may be NULL) receives 64
+ * ms-per-step figures, index = fused + 2 near_merged + 4 graphs +
+ * 8 xcd_chunk + 16 stream4 + 32 stream_grid.  Returns the chosen index or
+ * < 0.  Needs free device memory for one copy of the context. */
 int irbc_autotune(irbc_ctx *ctx, int32_t steps, double *report);
 /* keep per-cell results (irbc_get_cells) also for collisions run inside
  * irbc_md_steps; costs ~56 B per cell per collision */

@@ -22,6 +22,7 @@
 #include <iterator>
 #include <string>
 #include <map>
+#include <numeric>
 #include <vector>
 
 #include "../../include/irbc.h"
@@ -201,6 +202,8 @@ struct irbc_ctx {
     int near_merged = 0;        /* near-membrane streaming in one kernel (1) or sphere pass + hit pass (0) */
     int xcd_chunk = 1;          /* sort and cell kernels hand each XCD a contiguous eighth of the tiles */
     int stream4 = 0;            /* bulk streaming with four particles per lane */
+    int stream_grid = 0;        /* bulk streaming on a capped grid (IRBC_STREAM_WGS_PER_CU per CU) */
+    int ncu = 256;              /* compute units of the device */
     std::map<uint64_t, hipGraphExec_t> graphs;
     int nrep = 0;
     std::vector<HostRep> hr;
@@ -410,6 +413,9 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
         int lim = 65536;
         if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess) lim = 65536;
         c->mem_lds_nv = std::max(0, (std::min(lim, 65536) - 16384) / 32);
+        int ncu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+            c->ncu = ncu;
     }
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
@@ -1037,7 +1043,11 @@ static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t 
 }
 
 #define IRBC_GRAPH_STEPS 10 /* steps per captured chunk: one collision period */
-#define IRBC_TUNE_CANDIDATES 32 /* membrane shape x near shape x graphs x XCD chunking x stream shape */
+#define IRBC_TUNE_WARM_MAX 2100  /* most steps the autotuner runs to capture the chosen graphs */
+#define IRBC_TUNE_CANDIDATES 64 /* membrane shape x near shape x graphs x XCD chunking x stream shape x stream grid */
+#ifndef IRBC_STREAM_WGS_PER_CU
+#define IRBC_STREAM_WGS_PER_CU 4 /* capped bulk-streaming grid: workgroups per CU */
+#endif
 
 /* grid for a kernel that maps blocks through xcd_tile */
 static unsigned xcd_grid(unsigned n) { return IRBC_NXCD * ((n + IRBC_NXCD - 1) / IRBC_NXCD); }
@@ -1123,8 +1133,10 @@ static int launch_near(irbc_ctx *c)
 static int launch_bulk(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1)
 {
     const unsigned T = (unsigned)c->z.tiles;
+    const unsigned cap = (unsigned)(c->ncu * IRBC_STREAM_WGS_PER_CU);
     if (c->stream4) {
-        const unsigned G = (T + IRBC_WG / 64 - 1) / (IRBC_WG / 64);
+        unsigned G = (T + IRBC_WG / 64 - 1) / (IRBC_WG / 64);
+        if (c->stream_grid) G = std::min(G, cap);
         if (e0)
             hipExtLaunchKernelGGL(k_stream4, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep,
                                   (const int32_t *)c->d_tile_rep, (int)T, (const int32_t *)c->skey2,
@@ -1134,14 +1146,17 @@ static int launch_bulk(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1)
             k_stream4<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, c->skey2, c->ctab, c->lloff,
                                                 c->cdmin, c->nearq, c->par);
     } else {
+        const bool capped = c->stream_grid && T > cap;
+        const unsigned G = capped ? cap : T;
+        auto *kern = capped ? k_stream<true> : k_stream<false>;
         if (e0)
-            hipExtLaunchKernelGGL(k_stream, dim3(T), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep,
-                                  (const int32_t *)c->d_tile_rep, (const int32_t *)c->skey2,
+            hipExtLaunchKernelGGL(kern, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep,
+                                  (const int32_t *)c->d_tile_rep, (int)T, (const int32_t *)c->skey2,
                                   (const double *)c->ctab, (const int32_t *)c->lloff, (const float *)c->cdmin,
                                   c->nearq, c->par);
         else
-            k_stream<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab, c->lloff, c->cdmin,
-                                               c->nearq, c->par);
+            kern<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, c->skey2, c->ctab, c->lloff, c->cdmin,
+                                           c->nearq, c->par);
     }
     LAUNCHCHK();
     return IRBC_OK;
@@ -1418,6 +1433,7 @@ int irbc_autotune(irbc_ctx *c, int32_t steps, double *report)
         c->use_graphs = (k >> 2) & 1;
         c->xcd_chunk = (k >> 3) & 1;
         c->stream4 = (k >> 4) & 1;
+        c->stream_grid = (k >> 5) & 1;
         if (!rc) rc = snap_restore(c, S);
         if (!rc) rc = irbc_md_steps(c, steps); /* warm-up; also captures graphs */
         double ms = 0.0;
@@ -1441,15 +1457,27 @@ int irbc_autotune(irbc_ctx *c, int32_t steps, double *report)
             best_k = k;
         }
     }
-    int rc2 = snap_restore(c, S);
-    for (void *b : S.buf) hipFree(b);
-    if (rc) return rc;
-    if (rc2) return rc2;
     c->mem_fused = (best_k >> 0) & 1;
     c->near_merged = (best_k >> 1) & 1;
     c->use_graphs = (best_k >> 2) & 1;
     c->xcd_chunk = (best_k >> 3) & 1;
     c->stream4 = (best_k >> 4) & 1;
+    c->stream_grid = (best_k >> 5) & 1;
+    if (!rc && c->use_graphs) {
+        /* capture the chosen shape's graphs for one whole collide/list-update
+         * cycle now, so that later runs replay instead of capturing */
+        int64_t cyc = 20;
+        for (const HostRep &h : c->hr)
+            for (int64_t p : {(int64_t)h.spec.params.csteps, (int64_t)h.spec.params.uplist})
+                if (p > 0 && cyc <= IRBC_TUNE_WARM_MAX) cyc = cyc / std::gcd(cyc, p) * p;
+        rc = snap_restore(c, S);
+        if (!rc) rc = irbc_md_steps(c, std::min<int64_t>(cyc, IRBC_TUNE_WARM_MAX) + IRBC_GRAPH_STEPS);
+        if (!rc) rc = c->wait_all();
+    }
+    int rc2 = snap_restore(c, S);
+    for (void *b : S.buf) hipFree(b);
+    if (rc) return rc;
+    if (rc2) return rc2;
     return best_k;
 }
 
@@ -1758,7 +1786,7 @@ static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
     const uint64_t key = flags | ((uint64_t)(c->par & 1) << 48) | ((uint64_t)n << 50) |
                          ((uint64_t)(c->mem_fused & 1) << 58) | ((uint64_t)(c->diag_on & 1) << 59) |
                          ((uint64_t)(c->near_merged & 1) << 60) | ((uint64_t)(c->xcd_chunk & 1) << 61) |
-                         ((uint64_t)(c->stream4 & 1) << 62);
+                         ((uint64_t)(c->stream4 & 1) << 62) | ((uint64_t)(c->stream_grid & 1) << 63);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         const int par0 = c->par;
@@ -1851,6 +1879,9 @@ int irbc_set_option(irbc_ctx *c, int option, int value)
         return IRBC_OK;
     case IRBC_OPT_STREAM4:
         c->stream4 = value != 0;
+        return IRBC_OK;
+    case IRBC_OPT_STREAM_GRID:
+        c->stream_grid = value != 0;
         return IRBC_OK;
     default:
         return fail(IRBC_ERR_ARG, "irbc_set_option: unknown option %d", option);
