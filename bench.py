@@ -196,6 +196,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=2000)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the 64-replica batch against its first replica alone (for profiling)")
     ap.add_argument("--served-requests", type=int, default=24)
     ap.add_argument("--served-rate", type=float, default=4.0)
     ap.add_argument("--served-steps", type=int, default=5000)
@@ -209,6 +211,9 @@ def main():
               % a.gpus, file=sys.stderr)
         sys.exit(2)
     ia.load()
+    if a.batched_only:
+        print(json.dumps({"batched": batched(a.batched_steps, 100)}))
+        return
     d = en.single()
     b = ia.Batch([en.spec(d)])
     st = en.load(b, 0, d)
@@ -248,7 +253,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": load_traffic("k_stream_c2_traffic.md"),
-                     "kernel": "k_stream", "kernel_ms": k_ms, "kernel_launches_timed": prof["stream_n"]},
+                     "kernel": "k_stream4" if (tuned >> 4) & 1 else "k_stream", "kernel_ms": k_ms, "kernel_launches_timed": prof["stream_n"]},
         "launch_options": dict([(name, (tuned >> i) & 1) for i, (name, _) in enumerate(ia.OPTIONS)],
                                autotune_ms_per_step=[round(x, 4) for x in trials.tolist()]),
         "phases_ms_per_launch": {
