@@ -181,6 +181,7 @@ def served(n, rate, steps, budget_gb, plain_n=3):
                             "at density 5-10, 162/642/2562-vertex ellipsoids" % (n, rate, steps),
                 "hbm_budget_GiB": budget_gb, "chunk_steps": srv.chunk,
                 "micro_batches": srv.stats["groups"], "max_concurrent_micro_batches": srv.stats["max_active_groups"],
+                "max_resident_replicas": srv.stats["max_resident_replicas"],
                 "plain_replica_steps_per_s": s_plain / t_plain,
                 "plain_sample": "first %d requests run alone, one after another" % plain_n})
     out["speedup_vs_plain"] = out["replica_steps_per_s"] / out["plain_replica_steps_per_s"]
@@ -198,8 +199,8 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the 64-replica batch against its first replica alone (for profiling)")
-    ap.add_argument("--served-requests", type=int, default=24)
-    ap.add_argument("--served-rate", type=float, default=4.0)
+    ap.add_argument("--served-requests", type=int, default=64)
+    ap.add_argument("--served-rate", type=float, default=32.0)
     ap.add_argument("--served-steps", type=int, default=5000)
     ap.add_argument("--profile-every", type=int, default=4)
     ap.add_argument("--options", type=int, default=-1,

@@ -6,10 +6,14 @@ returns its rolling time series (membrane centroid, centroid velocity,
 bonded knobs every `series_every` steps) and its final membrane state.
 
 The loop
-  * admits arrived requests in arrival order while their device bytes
-    (irbc_estimate_bytes) fit the HBM budget left, grouping those that
-    arrive together into one context (a micro-batch: one launch sequence
-    for all of them);
+  * keeps at most `max_active` micro-batches on the GPU (each has its own
+    pair of HIP streams; more would share the device's few hardware queues
+    and serialise);
+  * when one of those places is free, admits every arrived request, in
+    arrival order, while their device bytes (irbc_estimate_bytes) fit the
+    HBM budget left, into one context (a micro-batch: one launch sequence
+    for all of them), so requests that arrive while the GPU is busy are
+    batched together;
   * keeps every admitted replica's state on the device until it is done;
   * advances every active context by `chunk` steps at a time, each on its
     own pair of HIP streams, so micro-batches admitted at different times
@@ -69,10 +73,11 @@ class _Group:
 
 
 class Server:
-    def __init__(self, budget_bytes, chunk=250, max_group=64):
+    def __init__(self, budget_bytes, chunk=250, max_group=64, max_active=2):
         self.budget = int(budget_bytes)
         self.chunk = int(chunk)
         self.max_group = int(max_group)
+        self.max_active = int(max_active)
         self.used = 0
         self.active = []
         self.stats = {"groups": 0, "max_active_groups": 0, "max_resident_replicas": 0}
@@ -81,7 +86,7 @@ class Server:
         """Arrived requests, in arrival order, while they fit the budget.
         Requests of one group share a step count (the group advances
         together); a request that does not fit waits for memory."""
-        while queue and queue[0].arrival <= now:
+        while queue and queue[0].arrival <= now and len(self.active) < self.max_active:
             head = queue[0]
             if head.bytes > self.budget:
                 raise ValueError("request %s needs %d bytes, more than the whole budget %d"
