@@ -194,7 +194,7 @@ struct irbc_ctx {
     int device = 0;
     hipStream_t st = nullptr;   /* solvent chain */
     hipStream_t st2 = nullptr;  /* membrane chain, overlapped with the solvent's streaming */
-    hipEvent_t ev_pre = nullptr, ev_mem = nullptr;
+    hipEvent_t ev_pre = nullptr, ev_mem2[2] = {nullptr, nullptr}, ev_str[2] = {nullptr, nullptr};
     int par = 0;                /* parity of the current step: picks the double-buffered state */
     int mem_fused = 0;          /* membrane step as one workgroup per replica (1) or five grid kernels (0) */
     int mem_lds_nv = 0;         /* largest mesh whose unwrapped positions the one-workgroup step keeps in LDS */
@@ -420,7 +420,10 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_pre, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_mem, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->ev_mem2[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_mem2[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_str[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_str[1], hipEventDisableTiming) != hipSuccess) {
         irbc_destroy(c);
         return fail(IRBC_ERR_HIP, "stream or event creation failed");
     }
@@ -525,8 +528,8 @@ int irbc_destroy(irbc_ctx *c)
         }
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
     for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
-    if (c->ev_pre) hipEventDestroy(c->ev_pre);
-    if (c->ev_mem) hipEventDestroy(c->ev_mem);
+    for (hipEvent_t e : {c->ev_pre, c->ev_mem2[0], c->ev_mem2[1], c->ev_str[0], c->ev_str[1]})
+        if (e) hipEventDestroy(e);
     if (c->st) hipStreamDestroy(c->st);
     if (c->st2) hipStreamDestroy(c->st2);
     delete c;
@@ -1180,7 +1183,7 @@ static int launch_tri_kin(irbc_ctx *c, int refresh)
         k_vkin<<<dim3((unsigned)((c->z.maxnv + IRBC_WG - 1) / IRBC_WG), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
             c->d_rep, M);
     k_tri_kin<<<dim3((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
-        c->d_rep, M);
+        c->d_rep, M, c->par & 1);
     LAUNCHCHK();
     return IRBC_OK;
 }
@@ -1688,18 +1691,32 @@ int irbc_checkpoint_load(irbc_ctx *c, int r, const void *buf, int64_t n)
 
 /* ---------------- the time loop ---------------- */
 
-/* Launches of one MD step.  Two chains: the solvent chain (collision, cell
- * lists, streaming, bounce-back) on c->st and the membrane chain on c->st2.
- * Streaming of step s only reads the triangle kinematics of step s, so the
- * membrane of step s (which writes those of step s+1 into the other
- * parity's buffer) runs beside it.  The step's last act is the solvent
- * stream waiting for the membrane: the collision of the next step moves
- * the vertices.  `rec` brackets the phases with timing events. */
-static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec)
+/* Launches of one MD step s (parity p).  Two chains: the solvent chain
+ * (collision, cell lists, triangle kinematics, streaming, bounce-back) on
+ * c->st and the membrane chain on c->st2.
+ *  - The solvent chain of step s waits for the membrane of step s-1: it
+ *    reads the vertex kinematics and the drift margin that membrane wrote
+ *    (parity p), and a collision or list update reads the vertices.
+ *  - The membrane of step s writes parity p^1 (vertex kinematics, drift
+ *    margin, next wall speed), last read by the solvent chain of step s-1
+ *    up to its bulk streaming.  So on a step without a collision or list
+ *    update it waits only for that streaming (ev_str of step s-1) and runs
+ *    beside the bounce-back of step s-1 and the solvent chain of step s.
+ *  - On a collision or list-update step, and on the first step of a run or
+ *    captured chunk, the membrane waits for this step's collision, lists and
+ *    cell reach distances (ev_pre).
+ * ev_mem and ev_str alternate by parity, so each wait binds to the step it
+ * means.  `first` skips the wait for the previous step's membrane (the
+ * caller has joined the chains), `rec` brackets phases with timing events.
+ * join_chains() makes the solvent stream wait for the last membrane. */
+static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec, bool first)
 {
     int rc;
+    const int p = c->par & 1;
     const int mem_phases = PH_ALL & ~PH_RESET_NEAR; /* k_stream clears the other parity's queue */
+    const bool lock = col || ll || first;
     hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!first) HIPCHK(hipStreamWaitEvent(c->st, c->ev_mem2[p ^ 1], 0));
     if (col) {
         if (rec) { e0 = c->event(); hipEventRecord(e0, c->st); }
         rc = launch_collision(c, 0, c->diag_on ? 1 : 0);
@@ -1710,23 +1727,21 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec)
         rc = launch_llist(c, 0);
         if (rc) return rc;
     }
-    /* this step's triangle kinematics (from the vertex kinematics the
-     * previous membrane gather or this collision wrote; the membrane chain
-     * writes the other parity's) and, for replicas that collided or made
-     * lists, the cell reach distances, which the membrane finish's drift
-     * margin builds on: then both before the membrane chain starts */
+    /* replicas that collided or made lists get new cell reach distances
+     * (from this step's triangle spheres), on which the membrane finish's
+     * drift margin builds */
     if (col || ll) {
         rc = launch_tri_kin(c, 0);
         if (rc) return rc;
         rc = launch_cell_reach(c, 0);
         if (rc) return rc;
     }
-    HIPCHK(hipEventRecord(c->ev_pre, c->st));
-    if (!col && !ll) {
-        rc = launch_tri_kin(c, 0);
-        if (rc) return rc;
+    if (lock) {
+        HIPCHK(hipEventRecord(c->ev_pre, c->st));
+        HIPCHK(hipStreamWaitEvent(c->st2, c->ev_pre, 0));
+    } else {
+        HIPCHK(hipStreamWaitEvent(c->st2, c->ev_str[p ^ 1], 0));
     }
-    HIPCHK(hipStreamWaitEvent(c->st2, c->ev_pre, 0));
     /* the two chains' launches alternate on the host, membrane first: the
      * host's launch latency then delays neither chain's start */
     hipEvent_t m0 = nullptr, s0 = nullptr;
@@ -1735,6 +1750,10 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec)
     int km = 0;
     rc = membrane_launch(c, mem_phases, c->st2, km++);
     if (rc) return rc;
+    if (!col && !ll) {
+        rc = launch_tri_kin(c, 0);
+        if (rc) return rc;
+    }
     if (rec) {
         /* events bound to the dispatch itself: the kernel's own duration */
         e0 = c->event();
@@ -1748,6 +1767,7 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec)
         rc = launch_bulk(c, nullptr, nullptr);
         if (rc) return rc;
     }
+    HIPCHK(hipEventRecord(c->ev_str[p], c->st));
     if (km < nm) {
         rc = membrane_launch(c, mem_phases, c->st2, km++);
         if (rc) return rc;
@@ -1760,9 +1780,16 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec)
         if (rc) return rc;
     }
     if (rec) { e1 = c->event(); hipEventRecord(e1, c->st2); c->pend[2].push_back({m0, e1}); }
-    HIPCHK(hipEventRecord(c->ev_mem, c->st2));
-    HIPCHK(hipStreamWaitEvent(c->st, c->ev_mem, 0));
+    HIPCHK(hipEventRecord(c->ev_mem2[p], c->st2));
     LAUNCHCHK();
+    return IRBC_OK;
+}
+
+/* the solvent stream waits for the membrane of the last launched step
+ * (parity par ^ 1): after it, work on c->st sees the whole step */
+static int join_chains(irbc_ctx *c)
+{
+    HIPCHK(hipStreamWaitEvent(c->st, c->ev_mem2[(c->par ^ 1) & 1], 0));
     return IRBC_OK;
 }
 
@@ -1793,9 +1820,10 @@ static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
         HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
         int rc = IRBC_OK;
         for (int k = 0; k < n && rc == IRBC_OK; ++k) {
-            rc = launch_step(c, (flags >> (2 * k)) & 1, (flags >> (2 * k + 1)) & 1, false);
+            rc = launch_step(c, (flags >> (2 * k)) & 1, (flags >> (2 * k + 1)) & 1, false, k == 0);
             c->par ^= 1;
         }
+        if (rc == IRBC_OK) rc = join_chains(c); /* the capture ends with both streams joined */
         hipGraph_t g = nullptr;
         const hipError_t ec = hipStreamEndCapture(c->st, &g);
         c->par = par0;
@@ -1850,7 +1878,7 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
         bool col, ll;
         next_flags(c, c->hr, col, ll);
         const bool rec = c->prof && (c->prof_step++ % c->prof_every) == 0;
-        rc = launch_step(c, col, ll, rec);
+        rc = launch_step(c, col, ll, rec, s == 0);
         if (rc) return rc;
         c->par ^= 1;
         if (c->prof && c->pend[0].size() >= 2048) {
@@ -1858,7 +1886,7 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
             if (rc) return rc;
         }
     }
-    return IRBC_OK;
+    return join_chains(c);
 }
 
 int irbc_set_option(irbc_ctx *c, int option, int value)
