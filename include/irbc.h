@@ -173,15 +173,22 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
  *   kernels running beside it; 0 (default): one workgroup per tile.  Same
  *   results. */
 #define IRBC_OPT_STREAM_GRID 6
+/* IRBC_OPT_MEM_AFTER_STREAM  1: the membrane part of a step starts after
+ *   that step's bulk streaming and overlaps the bounce-back; 0 (default): it
+ *   starts after the previous step's bulk streaming (on collision and list
+ *   steps after this step's collision and lists) and overlaps this step's
+ *   streaming.  Same results. */
+#define IRBC_OPT_MEM_AFTER_STREAM 7
 int irbc_set_option(irbc_ctx *ctx, int option, int value);
-/* Times `steps` MD steps under each of the 64 combinations of the options
+/* Times `steps` MD steps under each of the 128 combinations of the options
  * above (membrane shape, near shape, graphs, XCD chunking, stream shape,
- * stream grid) on this context's own replicas, then keeps the fastest.  The
- * state is snapshot on the device first and restored afterwards, so the
- * trajectory is not advanced.  report (may be NULL) receives 64
- * ms-per-step figures, index = fused + 2 near_merged + 4 graphs +
- * 8 xcd_chunk + 16 stream4 + 32 stream_grid.  Returns the chosen index or
- * < 0.  Needs free device memory for one copy of the context. */
+ * stream grid, membrane start) on this context's own replicas, then keeps
+ * the fastest.  The state is snapshot on the device first and restored
+ * afterwards, so the trajectory is not advanced.  report (may be NULL)
+ * receives 128 ms-per-step figures, index = fused + 2 near_merged +
+ * 4 graphs + 8 xcd_chunk + 16 stream4 + 32 stream_grid + 64 mem_after.
+ * Returns the chosen index or < 0.  Needs free device memory for one copy
+ * of the context. */
 int irbc_autotune(irbc_ctx *ctx, int32_t steps, double *report);
 /* keep per-cell results (irbc_get_cells) also for collisions run inside
  * irbc_md_steps; costs ~56 B per cell per collision */

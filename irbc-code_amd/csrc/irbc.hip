@@ -203,6 +203,7 @@ struct irbc_ctx {
     int xcd_chunk = 1;          /* sort and cell kernels hand each XCD a contiguous eighth of the tiles */
     int stream4 = 0;            /* bulk streaming with four particles per lane */
     int stream_grid = 0;        /* bulk streaming on a capped grid (IRBC_STREAM_WGS_PER_CU per CU) */
+    int mem_after = 0;          /* the membrane of a step starts after that step's bulk streaming */
     int ncu = 256;              /* compute units of the device */
     std::map<uint64_t, hipGraphExec_t> graphs;
     int nrep = 0;
@@ -1047,7 +1048,7 @@ static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t 
 
 #define IRBC_GRAPH_STEPS 10 /* steps per captured chunk: one collision period */
 #define IRBC_TUNE_WARM_MAX 2100  /* most steps the autotuner runs to capture the chosen graphs */
-#define IRBC_TUNE_CANDIDATES 64 /* membrane shape x near shape x graphs x XCD chunking x stream shape x stream grid */
+#define IRBC_TUNE_CANDIDATES 128 /* membrane shape x near shape x graphs x XCD chunking x stream shape x stream grid x membrane start */
 #ifndef IRBC_STREAM_WGS_PER_CU
 #define IRBC_STREAM_WGS_PER_CU 4 /* capped bulk-streaming grid: workgroups per CU */
 #endif
@@ -1437,6 +1438,7 @@ int irbc_autotune(irbc_ctx *c, int32_t steps, double *report)
         c->xcd_chunk = (k >> 3) & 1;
         c->stream4 = (k >> 4) & 1;
         c->stream_grid = (k >> 5) & 1;
+        c->mem_after = (k >> 6) & 1;
         if (!rc) rc = snap_restore(c, S);
         if (!rc) rc = irbc_md_steps(c, steps); /* warm-up; also captures graphs */
         double ms = 0.0;
@@ -1466,6 +1468,7 @@ int irbc_autotune(irbc_ctx *c, int32_t steps, double *report)
     c->xcd_chunk = (best_k >> 3) & 1;
     c->stream4 = (best_k >> 4) & 1;
     c->stream_grid = (best_k >> 5) & 1;
+    c->mem_after = (best_k >> 6) & 1;
     if (!rc && c->use_graphs) {
         /* capture the chosen shape's graphs for one whole collide/list-update
          * cycle now, so that later runs replay instead of capturing */
@@ -1736,20 +1739,26 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec, bool first)
         rc = launch_cell_reach(c, 0);
         if (rc) return rc;
     }
-    if (lock) {
-        HIPCHK(hipEventRecord(c->ev_pre, c->st));
-        HIPCHK(hipStreamWaitEvent(c->st2, c->ev_pre, 0));
-    } else {
-        HIPCHK(hipStreamWaitEvent(c->st2, c->ev_str[p ^ 1], 0));
-    }
-    /* the two chains' launches alternate on the host, membrane first: the
-     * host's launch latency then delays neither chain's start */
+    /* mem_after: the membrane waits for this step's bulk streaming instead
+     * (after the collision and lists too) and overlaps only the bounce-back,
+     * so it never shares HBM with the streaming sweep */
+    const bool after = c->mem_after != 0;
     hipEvent_t m0 = nullptr, s0 = nullptr;
-    if (rec) { m0 = c->event(); hipEventRecord(m0, c->st2); }
     const int nm = membrane_launches(c);
     int km = 0;
-    rc = membrane_launch(c, mem_phases, c->st2, km++);
-    if (rc) return rc;
+    if (!after) {
+        if (lock) {
+            HIPCHK(hipEventRecord(c->ev_pre, c->st));
+            HIPCHK(hipStreamWaitEvent(c->st2, c->ev_pre, 0));
+        } else {
+            HIPCHK(hipStreamWaitEvent(c->st2, c->ev_str[p ^ 1], 0));
+        }
+        /* the two chains' launches alternate on the host, membrane first:
+         * the host's launch latency then delays neither chain's start */
+        if (rec) { m0 = c->event(); hipEventRecord(m0, c->st2); }
+        rc = membrane_launch(c, mem_phases, c->st2, km++);
+        if (rc) return rc;
+    }
     if (!col && !ll) {
         rc = launch_tri_kin(c, 0);
         if (rc) return rc;
@@ -1768,6 +1777,10 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec, bool first)
         if (rc) return rc;
     }
     HIPCHK(hipEventRecord(c->ev_str[p], c->st));
+    if (after) {
+        HIPCHK(hipStreamWaitEvent(c->st2, c->ev_str[p], 0));
+        if (rec) { m0 = c->event(); hipEventRecord(m0, c->st2); }
+    }
     if (km < nm) {
         rc = membrane_launch(c, mem_phases, c->st2, km++);
         if (rc) return rc;
@@ -1813,7 +1826,8 @@ static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
     const uint64_t key = flags | ((uint64_t)(c->par & 1) << 48) | ((uint64_t)n << 50) |
                          ((uint64_t)(c->mem_fused & 1) << 58) | ((uint64_t)(c->diag_on & 1) << 59) |
                          ((uint64_t)(c->near_merged & 1) << 60) | ((uint64_t)(c->xcd_chunk & 1) << 61) |
-                         ((uint64_t)(c->stream4 & 1) << 62) | ((uint64_t)(c->stream_grid & 1) << 63);
+                         ((uint64_t)(c->stream4 & 1) << 62) | ((uint64_t)(c->stream_grid & 1) << 63) |
+                         ((uint64_t)(c->mem_after & 1) << 54);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         const int par0 = c->par;
@@ -1910,6 +1924,9 @@ int irbc_set_option(irbc_ctx *c, int option, int value)
         return IRBC_OK;
     case IRBC_OPT_STREAM_GRID:
         c->stream_grid = value != 0;
+        return IRBC_OK;
+    case IRBC_OPT_MEM_AFTER_STREAM:
+        c->mem_after = value != 0;
         return IRBC_OK;
     default:
         return fail(IRBC_ERR_ARG, "irbc_set_option: unknown option %d", option);

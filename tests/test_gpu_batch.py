@@ -160,15 +160,16 @@ def test_autotune_leaves_the_trajectory_alone():
 
 @pytest.mark.gpu
 def test_launch_options_do_not_change_results():
-    """Near-membrane shape, XCD chunking and the stream shape are pure
-    launch choices."""
+    """Near-membrane shape, XCD chunking, the stream shape and where the
+    membrane part of a step starts are pure launch choices."""
     ds = _mix()
     snaps = []
-    for near, xcd, s4 in ((1, 1, 0), (0, 1, 0), (1, 0, 0), (1, 1, 1)):
+    for near, xcd, s4, after in ((1, 1, 0, 0), (0, 1, 0, 0), (1, 0, 0, 0), (1, 1, 1, 0), (0, 1, 0, 1), (1, 1, 1, 1)):
         b = en.batch(ds)
         b.set_option(ia.OPT_NEAR_MERGED, near)
         b.set_option(ia.OPT_XCD_CHUNK, xcd)
         b.set_option(ia.OPT_STREAM4, s4)
+        b.set_option(ia.OPT_MEM_AFTER_STREAM, after)
         b.md_steps(31)
         b.sync()
         snaps.append([_snapshot(b, r) for r in range(len(ds))])
