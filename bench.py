@@ -137,16 +137,23 @@ def timed(b, steps, warmup):
     return time.perf_counter() - t0
 
 
-def batched(steps, warmup):
-    """64 replicas of 32x32x24 in one batch vs the first of them alone."""
+def set_options(b, k):
+    for i, (_, opt) in enumerate(ia.OPTIONS):
+        b.set_option(opt, (k >> i) & 1)
+    return k
+
+
+def batched(steps, warmup, options=-1):
+    """64 replicas of 32x32x24 in one batch vs the first of them alone
+    (each autotuned, or both with the given launch-option index)."""
     ds = en.uniform()
     b = en.batch(ds)
-    kb, _ = b.autotune(20)
+    kb = set_options(b, options) if options >= 0 else b.autotune(20)[0]
     dt_b = timed(b, steps, warmup)
     np_b = sum(en.n_solvent(d) for d in ds)
     b.close()
     a = en.batch(ds[:1])
-    ka, _ = a.autotune(50)
+    ka = set_options(a, options) if options >= 0 else a.autotune(50)[0]
     dt_a = timed(a, steps, warmup)
     a.close()
     rb = len(ds) * steps / dt_b
@@ -213,7 +220,7 @@ def main():
         sys.exit(2)
     ia.load()
     if a.batched_only:
-        print(json.dumps({"batched": batched(a.batched_steps, 100)}))
+        print(json.dumps({"batched": batched(a.batched_steps, 100, a.options)}))
         return
     d = en.single()
     b = ia.Batch([en.spec(d)])

@@ -327,8 +327,10 @@ struct irbc_ctx {
             ev_pool.pop_back();
             return e;
         }
+        /* timing only: no system-scope fence, so the event brackets the
+         * kernel and not a cache flush as well */
         hipEvent_t e;
-        hipEventCreate(&e);
+        hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
         return e;
     }
     int flush_prof()
