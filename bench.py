@@ -263,7 +263,7 @@ def main():
                      "traffic": load_traffic("k_stream_c2_traffic.md"),
                      "kernel": "k_stream4" if (tuned >> 4) & 1 else "k_stream", "kernel_ms": k_ms, "kernel_launches_timed": prof["stream_n"]},
         "launch_options": dict([(name, (tuned >> i) & 1) for i, (name, _) in enumerate(ia.OPTIONS)],
-                               autotune_ms_per_step=[round(x, 4) for x in trials.tolist()]),
+                               autotune_ms_per_step=[round(x, 4) if x == x else None for x in trials.tolist()]),
         "phases_ms_per_launch": {
             "stream": k_ms,
             "collision_sequence": prof["collision_ms"] / max(prof["collision_n"], 1),

@@ -179,14 +179,28 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
  *   steps after this step's collision and lists) and overlaps this step's
  *   streaming.  Same results. */
 #define IRBC_OPT_MEM_AFTER_STREAM 7
+/* IRBC_OPT_MEMBRANE_CHAIN  1 (default): the membrane part of a step is one
+ *   launch whose workgroups take their phase (begin, element terms, triangle
+ *   terms, gather, finish) by ticket and hand over through device-scope
+ *   release/acquire; it takes precedence over IRBC_OPT_MEMBRANE_FUSED.
+ *   0: IRBC_OPT_MEMBRANE_FUSED decides.  Same results as the five grid
+ *   kernels, bit for bit. */
+#define IRBC_OPT_MEMBRANE_CHAIN 8
+/* IRBC_OPT_CHAIN_DEVICE_SCOPE  1: the chain's phase hand-offs release and
+ *   acquire at device scope (L2 write-back and invalidation at each);
+ *   0 (default): XCD-local hand-offs (every workgroup of a replica runs on
+ *   one XCD, whose L2 is their point of coherence).  Same results. */
+#define IRBC_OPT_CHAIN_DEVICE_SCOPE 9
 int irbc_set_option(irbc_ctx *ctx, int option, int value);
-/* Times `steps` MD steps under each of the 128 combinations of the options
+/* Times `steps` MD steps under each of the 256 combinations of the options
  * above (membrane shape, near shape, graphs, XCD chunking, stream shape,
- * stream grid, membrane start) on this context's own replicas, then keeps
- * the fastest.  The state is snapshot on the device first and restored
- * afterwards, so the trajectory is not advanced.  report (may be NULL)
- * receives 128 ms-per-step figures, index = fused + 2 near_merged +
- * 4 graphs + 8 xcd_chunk + 16 stream4 + 32 stream_grid + 64 mem_after.
+ * stream grid, membrane start, membrane chain) on this context's own
+ * replicas, then keeps the fastest.  The state is snapshot on the device
+ * first and restored afterwards, so the trajectory is not advanced.  report
+ * (may be NULL) receives 256 ms-per-step figures, index = fused +
+ * 2 near_merged + 4 graphs + 8 xcd_chunk + 16 stream4 + 32 stream_grid +
+ * 64 mem_after + 128 mem_chain; indices with both the chain and the fused
+ * membrane set are not run (NaN: the chain takes precedence).
  * Returns the chosen index or < 0.  Needs free device memory for one copy
  * of the context. */
 int irbc_autotune(irbc_ctx *ctx, int32_t steps, double *report);

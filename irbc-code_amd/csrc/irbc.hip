@@ -204,6 +204,8 @@ struct irbc_ctx {
     int stream4 = 0;            /* bulk streaming with four particles per lane */
     int stream_grid = 0;        /* bulk streaming on a capped grid (IRBC_STREAM_WGS_PER_CU per CU) */
     int mem_after = 0;          /* the membrane of a step starts after that step's bulk streaming */
+    int mem_chain = 1;          /* the membrane step as one launch whose workgroups take their phase by ticket */
+    int chain_dev = 0;          /* its hand-offs at device scope (1) or XCD-local (0) */
     int ncu = 256;              /* compute units of the device */
     std::map<uint64_t, hipGraphExec_t> graphs;
     int nrep = 0;
@@ -1050,7 +1052,7 @@ static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t 
 
 #define IRBC_GRAPH_STEPS 10 /* steps per captured chunk: one collision period */
 #define IRBC_TUNE_WARM_MAX 2100  /* most steps the autotuner runs to capture the chosen graphs */
-#define IRBC_TUNE_CANDIDATES 128 /* membrane shape x near shape x graphs x XCD chunking x stream shape x stream grid x membrane start */
+#define IRBC_TUNE_CANDIDATES 256 /* membrane shape x near shape x graphs x XCD chunking x stream shape x stream grid x membrane start x membrane chain */
 #ifndef IRBC_STREAM_WGS_PER_CU
 #define IRBC_STREAM_WGS_PER_CU 4 /* capped bulk-streaming grid: workgroups per CU */
 #endif
@@ -1196,12 +1198,28 @@ static int launch_tri_kin(irbc_ctx *c, int refresh)
  * host (both chains then start at once).  The next step's vertex
  * kinematics go to the other parity's buffer.  Returns the number of
  * launches; launch k of them with membrane_launch(c, phases, s, k). */
-static int membrane_launches(const irbc_ctx *c) { return c->mem_fused ? 1 : 5; }
+static int membrane_launches(const irbc_ctx *c) { return (c->mem_chain || c->mem_fused) ? 1 : 5; }
+
+/* working workgroups per replica of k_mem_chain: its phases for the
+ * batch's largest mesh plus the finish */
+static unsigned chain_blocks(const irbc_ctx *c)
+{
+    const unsigned vb = (unsigned)((c->z.maxnv + IRBC_WG - 1) / IRBC_WG);
+    const unsigned eb = (unsigned)((std::max(c->z.maxne, c->z.maxnt) + IRBC_WG - 1) / IRBC_WG);
+    const unsigned tb = (unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG);
+    return vb + (eb + vb) + tb + vb + 1;
+}
 
 static int membrane_launch(irbc_ctx *c, int phases, hipStream_t s, int k)
 {
     const MemDev M = c->mem(c->par ^ 1);
     const unsigned nr = (unsigned)c->nrep;
+    if (c->mem_chain) {
+        auto *kern = c->chain_dev ? k_mem_chain<true> : k_mem_chain<false>;
+        kern<<<dim3(IRBC_NXCD * chain_blocks(c), nr), IRBC_WG, 0, s>>>(c->d_rep, phases, M, c->par);
+        LAUNCHCHK();
+        return IRBC_OK;
+    }
     if (c->mem_fused) {
         const int lds_nv = c->z.maxnv <= c->mem_lds_nv ? c->z.maxnv : 0;
         k_mem_step<<<nr, IRBC_MEM_WG, (size_t)32 * lds_nv, s>>>(c->d_rep, phases, M, c->par, lds_nv);
@@ -1434,6 +1452,13 @@ int irbc_autotune(irbc_ctx *c, int32_t steps, double *report)
     double best = 1e300;
     int best_k = 0;
     for (int k = 0; k < IRBC_TUNE_CANDIDATES; ++k) {
+        if (((k >> 7) & 1) && (k & 1)) {
+            /* the chain takes precedence over the one-workgroup membrane:
+             * the same launches as k - 1 */
+            if (report) report[k] = std::nan("");
+            continue;
+        }
+        c->mem_chain = (k >> 7) & 1;
         c->mem_fused = (k >> 0) & 1;
         c->near_merged = (k >> 1) & 1;
         c->use_graphs = (k >> 2) & 1;
@@ -1464,6 +1489,7 @@ int irbc_autotune(irbc_ctx *c, int32_t steps, double *report)
             best_k = k;
         }
     }
+    c->mem_chain = (best_k >> 7) & 1;
     c->mem_fused = (best_k >> 0) & 1;
     c->near_merged = (best_k >> 1) & 1;
     c->use_graphs = (best_k >> 2) & 1;
@@ -1829,7 +1855,7 @@ static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
                          ((uint64_t)(c->mem_fused & 1) << 58) | ((uint64_t)(c->diag_on & 1) << 59) |
                          ((uint64_t)(c->near_merged & 1) << 60) | ((uint64_t)(c->xcd_chunk & 1) << 61) |
                          ((uint64_t)(c->stream4 & 1) << 62) | ((uint64_t)(c->stream_grid & 1) << 63) |
-                         ((uint64_t)(c->mem_after & 1) << 54);
+                         ((uint64_t)(c->mem_after & 1) << 54) | ((uint64_t)(c->mem_chain & 1) << 55) | ((uint64_t)(c->chain_dev & 1) << 56);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         const int par0 = c->par;
@@ -1929,6 +1955,12 @@ int irbc_set_option(irbc_ctx *c, int option, int value)
         return IRBC_OK;
     case IRBC_OPT_MEM_AFTER_STREAM:
         c->mem_after = value != 0;
+        return IRBC_OK;
+    case IRBC_OPT_MEMBRANE_CHAIN:
+        c->mem_chain = value != 0;
+        return IRBC_OK;
+    case IRBC_OPT_CHAIN_DEVICE_SCOPE:
+        c->chain_dev = value != 0;
         return IRBC_OK;
     default:
         return fail(IRBC_ERR_ARG, "irbc_set_option: unknown option %d", option);

@@ -30,6 +30,7 @@
 #define IRBC_ST_CAND_OVERFLOW 2
 #define IRBC_ST_LLIST_OVERFLOW 4
 #define IRBC_ST_RANGE 8
+#define IRBC_ST_STALL 16       /* a bounded in-kernel wait ran out (the step's results are invalid) */
 
 /* counter-based stream ids (Philox4x32-10 counters: {index, stream, hi, lo}) */
 #define IRBC_RS_SHIFT 1u
@@ -100,6 +101,11 @@ struct DevRep {
     uint32_t vcur_bits, vnext_bits;
     double reach_acc[2], reach_drift[2];
     int32_t draw_total;     /* bond draws needed this step */
+    /* the one-launch membrane step: workgroup tickets of the running launch
+     * and the workgroups of each phase that have finished (all zero between
+     * launches: the finishing workgroup clears them) */
+    int32_t mem_ticket;
+    int32_t mem_done[4];
     /* time series: every `series_every` steps one record of IRBC_SERIES_REC
      * doubles (step, centroid x y z unwrapped, centroid velocity, bonds) */
     int32_t series_every, series_cap, series_n, series_pad;

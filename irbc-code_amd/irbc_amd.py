@@ -36,8 +36,10 @@ OPT_XCD_CHUNK = 4
 OPT_STREAM4 = 5
 OPT_STREAM_GRID = 6
 OPT_MEM_AFTER_STREAM = 7
+OPT_MEMBRANE_CHAIN = 8
+OPT_CHAIN_DEVICE_SCOPE = 9
 OPTIONS = (("membrane_fused", 1), ("near_merged", 3), ("graphs", 2), ("xcd_chunk", 4), ("stream4", 5),
-           ("stream_grid", 6), ("mem_after_stream", 7))
+           ("stream_grid", 6), ("mem_after_stream", 7), ("membrane_chain", 8))
 
 SERIES_FIELDS = ("step", "x", "y", "z", "vx", "vy", "vz", "bonds")
 
@@ -403,9 +405,9 @@ class Batch:
 
     def autotune(self, steps=20):
         """Pick the fastest launch options for this batch; returns
-        (chosen index, ms per step of each of the 128 candidates); bit i of
-        the index is option OPTIONS[i]."""
-        rep = np.zeros(128)
+        (chosen index, ms per step of each of the 256 candidates, NaN where
+        not run); bit i of the index is option OPTIONS[i]."""
+        rep = np.zeros(256)
         k = _chk(self.L.irbc_autotune(self.h, int(steps), _ptr(rep, _d)))
         return k, rep
 
