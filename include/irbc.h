@@ -101,8 +101,9 @@ int irbc_get_parasite(irbc_ctx *ctx, int r, double *d, int32_t *i);
 int64_t irbc_n_ligands(const irbc_ctx *ctx, int r);
 int irbc_set_ligands(irbc_ctx *ctx, int r, const int32_t *ligi);
 int irbc_get_ligands(irbc_ctx *ctx, int r, int32_t *ligi);
-/* g[10] = Sarea, Volume, Sarea0, Volume0, theta0, wallspeed, status,
- * bond draws of the last step, step, bonded knobs */
+/* g[11] = Sarea, Volume, Sarea0, Volume0, theta0, wallspeed, status,
+ * bond draws of the last step, step, bonded knobs, particles queued for
+ * bounce-back in the last step */
 int irbc_get_globals(irbc_ctx *ctx, int r, double *g);
 /* the time-loop counters before the top-of-step increment (maincode.cpp:450-475) */
 int irbc_set_counters(irbc_ctx *ctx, int r, int64_t step, int32_t colsteps, int32_t upd);
@@ -191,6 +192,11 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
  *   0 (default): XCD-local hand-offs (every workgroup of a replica runs on
  *   one XCD, whose L2 is their point of coherence).  Same results. */
 #define IRBC_OPT_CHAIN_DEVICE_SCOPE 9
+/* IRBC_OPT_NEAR_MASKS  1 (default): a particle goes to the bounce-back
+ *   kernels only when its 1/4-cell sub-cell lies near a listed triangle or
+ *   it travels far this step; 0: every particle of a cell that lists a
+ *   triangle goes (a check of the masks).  Same results. */
+#define IRBC_OPT_NEAR_MASKS 10
 int irbc_set_option(irbc_ctx *ctx, int option, int value);
 /* Times `steps` MD steps under each of the 256 combinations of the options
  * above (membrane shape, near shape, graphs, XCD chunking, stream shape,

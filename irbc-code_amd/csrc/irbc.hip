@@ -175,7 +175,7 @@ int64_t bytes_of(const Sizes &z, int n)
 {
     int64_t b = 0;
     b += z.slots * (2 * 6 * 4 + 2 * 4 + 4 * 4);           /* particles x2, ids x2, keys, ranks, sorted keys, queue */
-    b += z.cells * (4 + 8 * 8 + 4 + 4 + 8 * 8 + 4 + 4);   /* count, ctab, vhead, llcount, diag, cursor, reach */
+    b += z.cells * (4 + 8 * 8 + 4 + 4 + 8 * 8 + 4 + 8 + 4); /* count, ctab, vhead, llcount, diag, cursor, near mask, listed */
     b += z.cells * 8 * 8;                                 /* tapes */
     b += z.cells1 * (4 + 4);                              /* start, lloff */
     b += z.llent * 4;
@@ -206,6 +206,7 @@ struct irbc_ctx {
     int mem_after = 0;          /* the membrane of a step starts after that step's bulk streaming */
     int mem_chain = 1;          /* the membrane step as one launch whose workgroups take their phase by ticket */
     int chain_dev = 0;          /* its hand-offs at device scope (1) or XCD-local (0) */
+    int near_masks = 1;         /* sub-cell near masks (1) or every particle of a listed cell queued (0) */
     int ncu = 256;              /* compute units of the device */
     std::map<uint64_t, hipGraphExec_t> graphs;
     int nrep = 0;
@@ -227,7 +228,8 @@ struct irbc_ctx {
     /* cells */
     int32_t *ccount, *cstart, *vhead, *lloff, *llcount, *llent;
     double *ctab;
-    float *cdmin;
+    uint64_t *cmask;
+    int32_t *listed;  /* per replica: the cells that list a triangle, compacted */
     CellDiag diag;
     double *cell_u, *wall_z, *mem_z;
     /* membrane */
@@ -447,7 +449,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     A(c->ida, z.slots) A(c->idb, z.slots) A(c->skey, z.slots) A(c->srank, z.slots) A(c->skey2, z.slots) A(c->nearq, z.slots)
     A(c->deferq, 2 * z.slots)
     A(c->ccount, z.cells) A(c->cstart, z.cells1) A(c->vhead, z.cells) A(c->lloff, z.cells1)
-    A(c->llcount, z.cells) A(c->llent, z.llent) A(c->ctab, z.cells * 8) A(c->cdmin, z.cells)
+    A(c->llcount, z.cells) A(c->llent, z.llent) A(c->ctab, z.cells * 8) A(c->cmask, z.cells) A(c->listed, z.cells)
     A(c->diag.vcm, 3 * z.cells) A(c->diag.ncc, z.cells) A(c->diag.rand1, z.cells)
     A(c->diag.rand2, z.cells) A(c->diag.dvv, z.cells) A(c->diag.the, z.cells) A(c->diag.nc, z.cells)
     A(c->cell_u, 2 * z.cells) A(c->wall_z, 3 * z.cells) A(c->mem_z, 3 * z.cells)
@@ -914,6 +916,7 @@ int irbc_get_globals(irbc_ctx *c, int r, double *g)
     g[7] = R.bond_draws;
     g[8] = (double)R.step;
     g[9] = R.nbonds;
+    g[10] = R.near_count[c->par ^ 1];
     return IRBC_OK;
 }
 
@@ -1084,6 +1087,17 @@ static int launch_collision(irbc_ctx *c, int staged, int diag)
     return IRBC_OK;
 }
 
+/* the compacted list of cells that list a triangle, and cleared masks,
+ * for the replicas that made lists (all when staged) */
+static int launch_listed(irbc_ctx *c, int staged)
+{
+    k_listed_reset<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, staged);
+    k_cell_listed<<<(unsigned)c->z.ctiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, staged, c->lloff,
+                                                               c->listed, c->cmask);
+    LAUNCHCHK();
+    return IRBC_OK;
+}
+
 static int launch_llist(irbc_ctx *c, int staged)
 {
     int maxnt = 0;
@@ -1097,16 +1111,24 @@ static int launch_llist(irbc_ctx *c, int staged)
     k_tri_sort<<<(unsigned)c->z.ctiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, staged, c->lloff,
                                                             c->llcount, c->llent);
     LAUNCHCHK();
-    return IRBC_OK;
+    return launch_listed(c, staged);
 }
 
-/* the cell reach distances from the current lists and triangle spheres;
+/* the cells' near-membrane sub-cell masks from the current lists and
+ * triangle kinematics;
  * staged: every replica, else those that collide or make lists this step */
-static int launch_cell_reach(irbc_ctx *c, int staged)
+static int launch_cell_mask(irbc_ctx *c, int staged)
 {
-    k_cell_reach<<<(unsigned)c->z.ctiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, staged, c->par & 1,
-                                                              c->lloff, c->llent,
-                                                              c->tri_sph + (c->par & 1) * c->z.nt, c->cdmin);
+    if (staged) {
+        /* the lists may have come from elsewhere (a checkpoint, a stage call) */
+        const int rc = launch_listed(c, 1);
+        if (rc) return rc;
+    }
+    /* one wave per listed cell: about two and a half cells per triangle */
+    const unsigned G = (unsigned)std::max(16, c->z.maxnt * 5 / 8);
+    k_cell_mask<<<dim3(G, (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
+        c->d_rep, staged, c->par & 1, c->lloff, c->llent, c->listed, c->tri_kin + (c->par & 1) * c->tri_stride(),
+        c->tri_sph + (c->par & 1) * c->z.nt, c->cmask, c->near_masks);
     LAUNCHCHK();
     return IRBC_OK;
 }
@@ -1115,7 +1137,9 @@ static unsigned near_blocks(const irbc_ctx *c)
 {
     int64_t maxnp = 0;
     for (const DevRep &R : c->rep) maxnp = std::max<int64_t>(maxnp, R.np);
-    return (unsigned)std::max<int64_t>(16, (maxnp + 1023) / 1024);
+    /* the queue holds the particles of marked sub-cells, about 0.2-1% of a
+     * replica: a quarter of a percent per pass */
+    return (unsigned)std::max<int64_t>(16, (maxnp + 4095) / 4096);
 }
 
 static int launch_near(irbc_ctx *c)
@@ -1148,11 +1172,11 @@ static int launch_bulk(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1)
         if (e0)
             hipExtLaunchKernelGGL(k_stream4, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep,
                                   (const int32_t *)c->d_tile_rep, (int)T, (const int32_t *)c->skey2,
-                                  (const double *)c->ctab, (const int32_t *)c->lloff, (const float *)c->cdmin,
+                                  (const double *)c->ctab, (const int32_t *)c->lloff, (const uint64_t *)c->cmask,
                                   c->nearq, c->par);
         else
             k_stream4<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, c->skey2, c->ctab, c->lloff,
-                                                c->cdmin, c->nearq, c->par);
+                                                c->cmask, c->nearq, c->par);
     } else {
         const bool capped = c->stream_grid && T > cap;
         const unsigned G = capped ? cap : T;
@@ -1160,10 +1184,10 @@ static int launch_bulk(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1)
         if (e0)
             hipExtLaunchKernelGGL(kern, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep,
                                   (const int32_t *)c->d_tile_rep, (int)T, (const int32_t *)c->skey2,
-                                  (const double *)c->ctab, (const int32_t *)c->lloff, (const float *)c->cdmin,
+                                  (const double *)c->ctab, (const int32_t *)c->lloff, (const uint64_t *)c->cmask,
                                   c->nearq, c->par);
         else
-            kern<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, c->skey2, c->ctab, c->lloff, c->cdmin,
+            kern<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, c->skey2, c->ctab, c->lloff, c->cmask,
                                            c->nearq, c->par);
     }
     LAUNCHCHK();
@@ -1295,7 +1319,7 @@ int irbc_stream(irbc_ctx *c)
     if (rc) return rc;
     int rc2 = launch_tri_kin(c, 1);
     if (rc2) return rc2;
-    rc2 = launch_cell_reach(c, 1);
+    rc2 = launch_cell_mask(c, 1);
     if (rc2) return rc2;
     rc2 = launch_stream(c);
     if (rc2) return rc2;
@@ -1735,7 +1759,7 @@ int irbc_checkpoint_load(irbc_ctx *c, int r, const void *buf, int64_t n)
  *    beside the bounce-back of step s-1 and the solvent chain of step s.
  *  - On a collision or list-update step, and on the first step of a run or
  *    captured chunk, the membrane waits for this step's collision, lists and
- *    cell reach distances (ev_pre).
+ *    near masks (ev_pre).
  * ev_mem and ev_str alternate by parity, so each wait binds to the step it
  * means.  `first` skips the wait for the previous step's membrane (the
  * caller has joined the chains), `rec` brackets phases with timing events.
@@ -1758,13 +1782,13 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec, bool first)
         rc = launch_llist(c, 0);
         if (rc) return rc;
     }
-    /* replicas that collided or made lists get new cell reach distances
+    /* replicas that collided or made lists get new near masks
      * (from this step's triangle spheres), on which the membrane finish's
      * drift margin builds */
     if (col || ll) {
         rc = launch_tri_kin(c, 0);
         if (rc) return rc;
-        rc = launch_cell_reach(c, 0);
+        rc = launch_cell_mask(c, 0);
         if (rc) return rc;
     }
     /* mem_after: the membrane waits for this step's bulk streaming instead
@@ -1855,7 +1879,8 @@ static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
                          ((uint64_t)(c->mem_fused & 1) << 58) | ((uint64_t)(c->diag_on & 1) << 59) |
                          ((uint64_t)(c->near_merged & 1) << 60) | ((uint64_t)(c->xcd_chunk & 1) << 61) |
                          ((uint64_t)(c->stream4 & 1) << 62) | ((uint64_t)(c->stream_grid & 1) << 63) |
-                         ((uint64_t)(c->mem_after & 1) << 54) | ((uint64_t)(c->mem_chain & 1) << 55) | ((uint64_t)(c->chain_dev & 1) << 56);
+                         ((uint64_t)(c->mem_after & 1) << 54) | ((uint64_t)(c->mem_chain & 1) << 55) | ((uint64_t)(c->chain_dev & 1) << 56) |
+                         ((uint64_t)(c->near_masks & 1) << 57);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         const int par0 = c->par;
@@ -1898,7 +1923,7 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
     k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 1);
     rc = launch_tri_kin(c, 1);
     if (rc) return rc;
-    rc = launch_cell_reach(c, 1);
+    rc = launch_cell_mask(c, 1);
     if (rc) return rc;
     if (c->use_graphs && !c->prof) {
         const int CH = IRBC_GRAPH_STEPS;
@@ -1961,6 +1986,9 @@ int irbc_set_option(irbc_ctx *c, int option, int value)
         return IRBC_OK;
     case IRBC_OPT_CHAIN_DEVICE_SCOPE:
         c->chain_dev = value != 0;
+        return IRBC_OK;
+    case IRBC_OPT_NEAR_MASKS:
+        c->near_masks = value != 0;
         return IRBC_OK;
     default:
         return fail(IRBC_ERR_ARG, "irbc_set_option: unknown option %d", option);

@@ -41,8 +41,13 @@
 
 #define IRBC_SERIES_REC 8      /* doubles per time-series record */
 
-/* vertex drift up to which the cell reach distances stand (see k_cell_reach) */
+/* vertex drift up to which the cells' near masks stand (see k_cell_mask) */
 #define IRBC_DRIFT_MAX 0.25
+/* distance budget of the near masks: a sub-cell is marked when it lies
+ * within this of a listed triangle's sphere and slab; an unmarked
+ * particle streams freely while its step travel plus the vertex drift
+ * stays below it */
+#define IRBC_NEAR_D0 0.0625
 
 struct DevRep {
     /* ---- geometry and packing ---- */
@@ -93,10 +98,10 @@ struct DevRep {
     int32_t near_count[2];  /* particles queued for bounce-back, by step parity */
     int32_t defer_count[2]; /* of those, particles that cross a triangle plane */
     int32_t wca_count;      /* vertices inside the parasite's cutoff this step */
-    /* cell reach bookkeeping: float bits of the largest vertex speed after
+    /* near-mask drift bookkeeping: float bits of the largest vertex speed after
      * the first half-kick (a vertex moves that speed x dt in the step) of
      * this step and of the coming one; by step parity, the sum of those
-     * moves over the steps since the reach distances were made, before this
+     * moves over the steps since the near masks were made, before this
      * step, and the margin this step's streaming subtracts from them */
     uint32_t vcur_bits, vnext_bits;
     double reach_acc[2], reach_drift[2];
@@ -106,6 +111,7 @@ struct DevRep {
      * launches: the finishing workgroup clears them) */
     int32_t mem_ticket;
     int32_t mem_done[4];
+    int32_t nlisted;        /* cells that list a triangle (compacted at each list update) */
     /* time series: every `series_every` steps one record of IRBC_SERIES_REC
      * doubles (step, centroid x y z unwrapped, centroid velocity, bonds) */
     int32_t series_every, series_cap, series_n, series_pad;

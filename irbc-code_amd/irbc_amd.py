@@ -38,6 +38,7 @@ OPT_STREAM_GRID = 6
 OPT_MEM_AFTER_STREAM = 7
 OPT_MEMBRANE_CHAIN = 8
 OPT_CHAIN_DEVICE_SCOPE = 9
+OPT_NEAR_MASKS = 10
 OPTIONS = (("membrane_fused", 1), ("near_merged", 3), ("graphs", 2), ("xcd_chunk", 4), ("stream4", 5),
            ("stream_grid", 6), ("mem_after_stream", 7), ("membrane_chain", 8))
 
@@ -295,10 +296,10 @@ class Batch:
         _chk(self.L.irbc_set_ligands(self.h, r, _ptr(a, _i)))
 
     def globals(self, r=0):
-        g = np.zeros(10)
+        g = np.zeros(11)
         _chk(self.L.irbc_get_globals(self.h, r, _ptr(g, _d)))
         return dict(zip(("Sarea", "Volume", "Sarea0", "Volume0", "theta0", "wallspeed", "status",
-                         "bond_draws", "step", "nbonds"), g))
+                         "bond_draws", "step", "nbonds", "near_queue"), g))
 
     def set_counters(self, r, step, colsteps, upd):
         _chk(self.L.irbc_set_counters(self.h, r, int(step), int(colsteps), int(upd)))

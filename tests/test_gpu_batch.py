@@ -165,6 +165,32 @@ def test_membrane_chain_equals_grid_kernels(graphs, dev):
 
 
 @pytest.mark.gpu
+def test_near_masks_never_change_results():
+    """The sub-cell near masks only decide which particles go to the
+    bounce-back kernels: with them off (every particle of a listed cell
+    queued) the trajectories are the same bits over 120 steps (12
+    collisions, list updates), for fast walls and fast-moving membranes too,
+    and with them on the queue is shorter."""
+    ds = _mix() + [en.replica(20, 20, 16, 10, 642, None, 1.0, 300, kon=5.0, koff=1.0, shear=0.25, seed=31)]
+    snaps, queues = [], []
+    for masks in (0, 1):
+        b = en.batch(ds)
+        b.set_option(ia.OPT_NEAR_MASKS, masks)
+        q = np.zeros(len(ds))
+        for _ in range(12):
+            b.md_steps(10)
+            b.sync()
+            q += [b.globals(r)["near_queue"] for r in range(len(ds))]
+        snaps.append([_snapshot(b, r) for r in range(len(ds))])
+        queues.append(q)
+        b.close()
+    for r in range(len(ds)):
+        for key in snaps[0][r]:
+            np.testing.assert_array_equal(snaps[0][r][key], snaps[1][r][key], err_msg="replica %d %s" % (r, key))
+    assert np.all(queues[1] < queues[0]), queues
+
+
+@pytest.mark.gpu
 def test_autotune_leaves_the_trajectory_alone():
     """The autotuner's trial steps are undone: a batch tuned and then run
     equals a batch run with the chosen options from the start."""
