@@ -197,6 +197,12 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
  *   it travels far this step; 0: every particle of a cell that lists a
  *   triangle goes (a check of the masks).  Same results. */
 #define IRBC_OPT_NEAR_MASKS 10
+/* IRBC_OPT_ONE_QUEUE  1 (default): with the membrane chain, a step is two
+ *   launches on one queue -- the chain's force phases, the triangle
+ *   kinematics and the bulk streaming in one grid, the bounce-back and the
+ *   membrane finish in the next; 0: the membrane on a second HIP stream with
+ *   events between the two.  Same results. */
+#define IRBC_OPT_ONE_QUEUE 11
 int irbc_set_option(irbc_ctx *ctx, int option, int value);
 /* Times `steps` MD steps under each of the 256 combinations of the options
  * above (membrane shape, near shape, graphs, XCD chunking, stream shape,

@@ -30,6 +30,7 @@
 
 #include "k_solvent.inc"
 #include "k_membrane.inc"
+#include "k_step.inc"
 
 namespace {
 
@@ -207,6 +208,7 @@ struct irbc_ctx {
     int mem_chain = 1;          /* the membrane step as one launch whose workgroups take their phase by ticket */
     int chain_dev = 0;          /* its hand-offs at device scope (1) or XCD-local (0) */
     int near_masks = 1;         /* sub-cell near masks (1) or every particle of a listed cell queued (0) */
+    int one_queue = 1;          /* with the chain: the step on one queue, the membrane inside the solvent's launches */
     int ncu = 256;              /* compute units of the device */
     std::map<uint64_t, hipGraphExec_t> graphs;
     int nrep = 0;
@@ -1161,34 +1163,37 @@ static int launch_near(irbc_ctx *c)
 }
 
 /* the bulk streaming kernel in the chosen shape; e0/e1 (both or neither)
- * are events bound to the dispatch */
-static int launch_bulk(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1)
+ * are events bound to the dispatch; with tri_kin its first blocks write
+ * this step's triangle kinematics from the vertex kinematics */
+static int launch_bulk(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1, bool tri_kin)
 {
     const unsigned T = (unsigned)c->z.tiles;
     const unsigned cap = (unsigned)(c->ncu * IRBC_STREAM_WGS_PER_CU);
+    const MemDev M = c->mem(c->par);
+    const TriKin tk{M.t1, M.t2, M.t3, M.vkin, M.tri_kin, M.tri_sph};
+    const int tkb = std::max(1, (c->z.maxnt + IRBC_WG - 1) / IRBC_WG);
+    const int ntk = tri_kin ? tkb * c->nrep : 0;
+    const int32_t *tr = c->d_tile_rep, *sk = c->skey2, *lo = c->lloff;
+    const double *ct = c->ctab;
+    const uint64_t *cm = c->cmask;
     if (c->stream4) {
         unsigned G = (T + IRBC_WG / 64 - 1) / (IRBC_WG / 64);
         if (c->stream_grid) G = std::min(G, cap);
+        G += (unsigned)ntk;
         if (e0)
-            hipExtLaunchKernelGGL(k_stream4, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep,
-                                  (const int32_t *)c->d_tile_rep, (int)T, (const int32_t *)c->skey2,
-                                  (const double *)c->ctab, (const int32_t *)c->lloff, (const uint64_t *)c->cmask,
-                                  c->nearq, c->par);
+            hipExtLaunchKernelGGL(k_stream4, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep, tr, (int)T, sk, ct,
+                                  lo, cm, c->nearq, c->par, tk, tkb, ntk);
         else
-            k_stream4<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, c->skey2, c->ctab, c->lloff,
-                                                c->cmask, c->nearq, c->par);
+            k_stream4<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, tr, (int)T, sk, ct, lo, cm, c->nearq, c->par, tk, tkb, ntk);
     } else {
         const bool capped = c->stream_grid && T > cap;
-        const unsigned G = capped ? cap : T;
+        const unsigned G = (capped ? cap : T) + (unsigned)ntk;
         auto *kern = capped ? k_stream<true> : k_stream<false>;
         if (e0)
-            hipExtLaunchKernelGGL(kern, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep,
-                                  (const int32_t *)c->d_tile_rep, (int)T, (const int32_t *)c->skey2,
-                                  (const double *)c->ctab, (const int32_t *)c->lloff, (const uint64_t *)c->cmask,
-                                  c->nearq, c->par);
+            hipExtLaunchKernelGGL(kern, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep, tr, (int)T, sk, ct, lo,
+                                  cm, c->nearq, c->par, tk, tkb, ntk);
         else
-            kern<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, c->skey2, c->ctab, c->lloff, c->cmask,
-                                           c->nearq, c->par);
+            kern<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, tr, (int)T, sk, ct, lo, cm, c->nearq, c->par, tk, tkb, ntk);
     }
     LAUNCHCHK();
     return IRBC_OK;
@@ -1196,7 +1201,7 @@ static int launch_bulk(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1)
 
 static int launch_stream(irbc_ctx *c)
 {
-    int rc = launch_bulk(c, nullptr, nullptr);
+    int rc = launch_bulk(c, nullptr, nullptr, false);
     if (rc) return rc;
     return launch_near(c);
 }
@@ -1212,7 +1217,7 @@ static int launch_tri_kin(irbc_ctx *c, int refresh)
         k_vkin<<<dim3((unsigned)((c->z.maxnv + IRBC_WG - 1) / IRBC_WG), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
             c->d_rep, M);
     k_tri_kin<<<dim3((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
-        c->d_rep, M, c->par & 1);
+        c->d_rep, M);
     LAUNCHCHK();
     return IRBC_OK;
 }
@@ -1811,21 +1816,20 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec, bool first)
         rc = membrane_launch(c, mem_phases, c->st2, km++);
         if (rc) return rc;
     }
-    if (!col && !ll) {
-        rc = launch_tri_kin(c, 0);
-        if (rc) return rc;
-    }
+    /* the triangle kinematics of a step without collision or lists come
+     * with the streaming launch (its first blocks) */
+    const bool tk = !col && !ll;
     if (rec) {
         /* events bound to the dispatch itself: the kernel's own duration */
         e0 = c->event();
         e1 = c->event();
-        rc = launch_bulk(c, e0, e1);
+        rc = launch_bulk(c, e0, e1, tk);
         if (rc) return rc;
         c->pend[0].push_back({e0, e1});
         s0 = c->event();
         hipEventRecord(s0, c->st);
     } else {
-        rc = launch_bulk(c, nullptr, nullptr);
+        rc = launch_bulk(c, nullptr, nullptr, tk);
         if (rc) return rc;
     }
     HIPCHK(hipEventRecord(c->ev_str[p], c->st));
@@ -1850,10 +1854,108 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec, bool first)
     return IRBC_OK;
 }
 
+/* Is the step run on one queue (k_step_stream + k_step_near)?  Needs the
+ * one-launch membrane chain. */
+static bool one_queue(const irbc_ctx *c) { return c->one_queue && c->mem_chain; }
+
+/* The membrane chain (phases up to the gather) of this step, this step's
+ * triangle kinematics (tri_kin) and its bulk streaming in one launch;
+ * e0/e1 (both or neither) are events bound to the dispatch. */
+static int launch_step_stream(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1, bool tri_kin, int phases)
+{
+    const MemDev Mm = c->mem(c->par ^ 1); /* the membrane writes the next step's parity */
+    const MemDev Ms = c->mem(c->par);
+    const TriKin tk{Ms.t1, Ms.t2, Ms.t3, Ms.vkin, Ms.tri_kin, Ms.tri_sph};
+    const int NB = (int)chain_blocks(c) - 1; /* no finish ticket: the finish runs in k_step_near */
+    const int CB = IRBC_NXCD * NB * ((c->nrep + IRBC_NXCD - 1) / IRBC_NXCD);
+    const int tkb = std::max(1, (c->z.maxnt + IRBC_WG - 1) / IRBC_WG);
+    const int ntk = tri_kin ? tkb * c->nrep : 0;
+    const int T = (int)c->z.tiles;
+    const int32_t *tr = c->d_tile_rep, *sk = c->skey2, *lo = c->lloff;
+    const double *ct = c->ctab;
+    const uint64_t *cm = c->cmask;
+    const bool four = c->stream4 != 0;
+    const unsigned G = (unsigned)(CB + ntk + (four ? (T + IRBC_WG / 64 - 1) / (IRBC_WG / 64) : T));
+    auto *kern = four ? k_step_stream<true> : k_step_stream<false>;
+    if (e0)
+        hipExtLaunchKernelGGL(kern, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep, c->nrep, phases, Mm, NB,
+                              CB, tr, T, sk, ct, lo, cm, c->nearq, c->par, tk, tkb, ntk);
+    else
+        kern<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, c->nrep, phases, Mm, NB, CB, tr, T, sk, ct, lo, cm, c->nearq, c->par,
+                                       tk, tkb, ntk);
+    LAUNCHCHK();
+    return IRBC_OK;
+}
+
+/* this step's bounce-back and the membrane finish (one block per replica) */
+static int launch_step_near(irbc_ctx *c, int phases)
+{
+    const double *tk = c->tri_kin + (c->par & 1) * c->tri_stride();
+    const float4 *ts = c->tri_sph + (c->par & 1) * c->z.nt;
+    const MemDev Mm = c->mem(c->par ^ 1);
+    const unsigned nr = (unsigned)c->nrep;
+    if (c->near_merged) {
+        k_step_near<true><<<dim3(near_blocks(c) + 1, nr), IRBC_WG, 0, c->st>>>(c->d_rep, c->lloff, c->llent,
+                                                                           c->nearq, tk, ts, c->par, phases, Mm);
+    } else {
+        k_near_sphere<<<dim3(std::max(4u, near_blocks(c) / 8u), nr), IRBC_WG, 0, c->st>>>(
+            c->d_rep, c->lloff, c->llent, c->nearq, c->deferq, tk, ts, c->par);
+        k_step_near<false><<<dim3(std::max(16u, near_blocks(c) / 8u) + 1, nr), IRBC_WG, 0, c->st>>>(
+            c->d_rep, c->lloff, c->llent, c->deferq, tk, ts, c->par, phases, Mm);
+    }
+    LAUNCHCHK();
+    return IRBC_OK;
+}
+
+/* One MD step on one queue: [collision] [lists] [triangle kinematics and
+ * near masks after either], the fused streaming launch, the fused
+ * bounce-back launch.  rec brackets phases with timing events (the
+ * membrane runs inside the streaming launch and has no figure of its own). */
+static int launch_step_one(irbc_ctx *c, bool col, bool ll, bool rec)
+{
+    int rc;
+    const int mem_phases = PH_ALL & ~PH_RESET_NEAR; /* k_stream clears the other parity's queue */
+    hipEvent_t e0 = nullptr, e1 = nullptr, s0 = nullptr;
+    if (col) {
+        if (rec) { e0 = c->event(); hipEventRecord(e0, c->st); }
+        rc = launch_collision(c, 0, c->diag_on ? 1 : 0);
+        if (rc) return rc;
+        if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[1].push_back({e0, e1}); }
+    }
+    if (ll) {
+        rc = launch_llist(c, 0);
+        if (rc) return rc;
+    }
+    if (col || ll) {
+        rc = launch_tri_kin(c, 0);
+        if (rc) return rc;
+        rc = launch_cell_mask(c, 0);
+        if (rc) return rc;
+    }
+    const bool tk = !col && !ll;
+    if (rec) {
+        e0 = c->event();
+        e1 = c->event();
+        rc = launch_step_stream(c, e0, e1, tk, mem_phases);
+        if (rc) return rc;
+        c->pend[0].push_back({e0, e1});
+        s0 = c->event();
+        hipEventRecord(s0, c->st);
+    } else {
+        rc = launch_step_stream(c, nullptr, nullptr, tk, mem_phases);
+        if (rc) return rc;
+    }
+    rc = launch_step_near(c, mem_phases);
+    if (rc) return rc;
+    if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[3].push_back({s0, e1}); }
+    return IRBC_OK;
+}
+
 /* the solvent stream waits for the membrane of the last launched step
  * (parity par ^ 1): after it, work on c->st sees the whole step */
 static int join_chains(irbc_ctx *c)
 {
+    if (one_queue(c)) return IRBC_OK;
     HIPCHK(hipStreamWaitEvent(c->st, c->ev_mem2[(c->par ^ 1) & 1], 0));
     return IRBC_OK;
 }
@@ -1880,14 +1982,16 @@ static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
                          ((uint64_t)(c->near_merged & 1) << 60) | ((uint64_t)(c->xcd_chunk & 1) << 61) |
                          ((uint64_t)(c->stream4 & 1) << 62) | ((uint64_t)(c->stream_grid & 1) << 63) |
                          ((uint64_t)(c->mem_after & 1) << 54) | ((uint64_t)(c->mem_chain & 1) << 55) | ((uint64_t)(c->chain_dev & 1) << 56) |
-                         ((uint64_t)(c->near_masks & 1) << 57);
+                         ((uint64_t)(c->near_masks & 1) << 57) |
+                         ((uint64_t)(c->one_queue & 1) << 53);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         const int par0 = c->par;
         HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
         int rc = IRBC_OK;
         for (int k = 0; k < n && rc == IRBC_OK; ++k) {
-            rc = launch_step(c, (flags >> (2 * k)) & 1, (flags >> (2 * k + 1)) & 1, false, k == 0);
+            const bool col = (flags >> (2 * k)) & 1, ll = (flags >> (2 * k + 1)) & 1;
+            rc = one_queue(c) ? launch_step_one(c, col, ll, false) : launch_step(c, col, ll, false, k == 0);
             c->par ^= 1;
         }
         if (rc == IRBC_OK) rc = join_chains(c); /* the capture ends with both streams joined */
@@ -1945,7 +2049,7 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
         bool col, ll;
         next_flags(c, c->hr, col, ll);
         const bool rec = c->prof && (c->prof_step++ % c->prof_every) == 0;
-        rc = launch_step(c, col, ll, rec, s == 0);
+        rc = one_queue(c) ? launch_step_one(c, col, ll, rec) : launch_step(c, col, ll, rec, s == 0);
         if (rc) return rc;
         c->par ^= 1;
         if (c->prof && c->pend[0].size() >= 2048) {
@@ -1989,6 +2093,9 @@ int irbc_set_option(irbc_ctx *c, int option, int value)
         return IRBC_OK;
     case IRBC_OPT_NEAR_MASKS:
         c->near_masks = value != 0;
+        return IRBC_OK;
+    case IRBC_OPT_ONE_QUEUE:
+        c->one_queue = value != 0;
         return IRBC_OK;
     default:
         return fail(IRBC_ERR_ARG, "irbc_set_option: unknown option %d", option);

@@ -82,10 +82,8 @@ struct DevRep {
     int32_t collide, llist; /* this step runs a collision / a list update */
     int32_t rotate_pending; /* collision done, solvent rotation fused into the next stream */
     int32_t cur;            /* which particle buffer holds the state (0 = a, 1 = b) */
-    double wall2[2];        /* top-wall speed of this and the next step, by step parity */
-    double wall_cur;        /* this step's top-wall speed as the streaming kernels read it: copied
-                             * by the triangle kinematics at the top of the solvent chain, so the
-                             * membrane chain may already write the next parity's */
+    double wall2[2];        /* top-wall speed of this and the next step, by step parity (the
+                             * membrane part of step s writes that of step s+1) */
     double shift[3];
     double Sarea, Volume;
     double para[3], paraV[3], paraft[3], parafdt[3];

@@ -138,11 +138,13 @@ def test_fused_membrane_with_lds_positions_far_into_the_batch():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("graphs,dev", [(0, 0), (1, 0), (1, 1)])
-def test_membrane_chain_equals_grid_kernels(graphs, dev):
-    """The one-launch membrane step (workgroups ordered by ticket), with
-    XCD-local or device-scope hand-offs, runs the five grid kernels'
-    arithmetic element by element: same bits, in an
+@pytest.mark.parametrize("graphs,dev,oneq,four", [(0, 0, 1, 0), (1, 0, 1, 0), (1, 0, 1, 1), (1, 0, 0, 0), (1, 1, 0, 0)])
+def test_membrane_chain_equals_grid_kernels(graphs, dev, oneq, four):
+    """The one-launch membrane step (workgroups ordered by ticket) -- inside
+    the streaming and bounce-back launches of a one-queue step (either
+    stream shape), or on its own stream with XCD-local or device-scope
+    hand-offs -- runs the five grid kernels' arithmetic element by element
+    (those on their own stream): same bits, in an
     uneven batch whose smaller meshes leave some of the launch's tickets
     idle, over steps with collisions, list updates and bond events."""
     ds = _mix() + [en.replica(16, 16, 16, 6, 642, None, 1.0, 300, kon=100.0, koff=10.0, shear=0.05, seed=21)]
@@ -153,6 +155,8 @@ def test_membrane_chain_equals_grid_kernels(graphs, dev):
         b.set_option(ia.OPT_MEMBRANE_FUSED, 0)
         b.set_option(ia.OPT_MEMBRANE_CHAIN, chain)
         b.set_option(ia.OPT_CHAIN_DEVICE_SCOPE, dev)
+        b.set_option(ia.OPT_ONE_QUEUE, oneq)
+        b.set_option(ia.OPT_STREAM4, four)
         for n in (13, 40):
             b.md_steps(n)
             b.sync()
