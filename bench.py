@@ -36,7 +36,11 @@ import irbc_amd as ia  # noqa: E402
 import serve  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-STREAM_BYTES_PER_PARTICLE = 48.0  # x,y,z,vx,vy,vz FP32 read + write
+# algorithmic bytes per particle per streaming launch: x,y,z,vx,vy,vz FP32
+# read and x,y,z written (36 B); on the step after a collision the rotated
+# velocities are written and the sorted cell keys read (16 B more), once
+# every csteps = 10 launches
+STREAM_BYTES_PER_PARTICLE = 36.0 + 16.0 / 10.0
 
 
 def cpu_baseline(b, d, st, budget_s=15.0, max_steps=5000):
@@ -249,6 +253,12 @@ def main():
     value = ns * a.steps / dt
     k_ms = prof["stream_ms"] / max(prof["stream_n"], 1)
     achieved = STREAM_BYTES_PER_PARTICLE * ns / (k_ms * 1e-3) / 1e9
+    opt = dict((name, (tuned >> i) & 1) for i, (name, _) in enumerate(ia.OPTIONS))
+    if opt["one_queue"] and opt["membrane_chain"]:
+        kname = "k_step_stream<%s>" % ("true" if opt["stream4"] else "false")
+    else:
+        kname = "k_stream4" if opt["stream4"] else "k_stream<false>"
+    tfile = "%s_c2_traffic.md" % kname.replace("<", "_").replace(">", "")
     line = {
         "metric": "MPCD particle-steps/s", "value": value, "unit": "particle-steps/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": dt * 1e3 / a.steps,
@@ -260,14 +270,15 @@ def main():
                    "replicas": 1, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": load_traffic("k_stream_c2_traffic.md"),
-                     "kernel": "k_stream4" if (tuned >> 4) & 1 else "k_stream", "kernel_ms": k_ms, "kernel_launches_timed": prof["stream_n"]},
+                     "traffic": load_traffic(tfile), "bytes_per_particle": STREAM_BYTES_PER_PARTICLE,
+                     "kernel": kname, "kernel_ms": k_ms, "kernel_launches_timed": prof["stream_n"]},
         "launch_options": dict([(name, (tuned >> i) & 1) for i, (name, _) in enumerate(ia.OPTIONS)],
                                autotune_ms_per_step=[round(x, 4) if x == x else None for x in trials.tolist()]),
         "phases_ms_per_launch": {
             "stream": k_ms,
             "collision_sequence": prof["collision_ms"] / max(prof["collision_n"], 1),
-            "membrane": prof["membrane_ms"] / max(prof["membrane_n"], 1),
+            # on one queue the membrane runs inside the streaming launch
+            "membrane": prof["membrane_ms"] / prof["membrane_n"] if prof["membrane_n"] else None,
             "stream_near": prof["near_ms"] / max(prof["near_n"], 1)},
     }
     if not a.no_cpu:

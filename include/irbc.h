@@ -204,17 +204,16 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
  *   events between the two.  Same results. */
 #define IRBC_OPT_ONE_QUEUE 11
 int irbc_set_option(irbc_ctx *ctx, int option, int value);
-/* Times `steps` MD steps under each of the 256 combinations of the options
- * above (membrane shape, near shape, graphs, XCD chunking, stream shape,
- * stream grid, membrane start, membrane chain) on this context's own
- * replicas, then keeps the fastest.  The state is snapshot on the device
- * first and restored afterwards, so the trajectory is not advanced.  report
- * (may be NULL) receives 256 ms-per-step figures, index = fused +
- * 2 near_merged + 4 graphs + 8 xcd_chunk + 16 stream4 + 32 stream_grid +
- * 64 mem_after + 128 mem_chain; indices with both the chain and the fused
- * membrane set are not run (NaN: the chain takes precedence).
- * Returns the chosen index or < 0.  Needs free device memory for one copy
- * of the context. */
+/* Times `steps` MD steps under 48 combinations of the options above on
+ * this context's own replicas, then keeps the fastest: the membrane as the
+ * chain on one queue, the chain on its own stream or the five grid kernels
+ * on its own stream, times bounce-back shape, graphs, XCD chunking and
+ * stream shape.  The state is snapshot on the device first and restored
+ * afterwards, so the trajectory is not advanced.  report (may be NULL)
+ * receives 512 ms-per-step figures, index = fused + 2 near_merged +
+ * 4 graphs + 8 xcd_chunk + 16 stream4 + 32 stream_grid + 64 mem_after +
+ * 128 mem_chain + 256 one_queue, NaN where not timed.  Returns the chosen
+ * index or < 0.  Needs free device memory for one copy of the context. */
 int irbc_autotune(irbc_ctx *ctx, int32_t steps, double *report);
 /* keep per-cell results (irbc_get_cells) also for collisions run inside
  * irbc_md_steps; costs ~56 B per cell per collision */

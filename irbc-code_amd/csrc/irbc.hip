@@ -1057,7 +1057,7 @@ static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t 
 
 #define IRBC_GRAPH_STEPS 10 /* steps per captured chunk: one collision period */
 #define IRBC_TUNE_WARM_MAX 2100  /* most steps the autotuner runs to capture the chosen graphs */
-#define IRBC_TUNE_CANDIDATES 256 /* membrane shape x near shape x graphs x XCD chunking x stream shape x stream grid x membrane start x membrane chain */
+#define IRBC_TUNE_CANDIDATES 512 /* index space: 9 option bits (irbc_autotune) */
 #ifndef IRBC_STREAM_WGS_PER_CU
 #define IRBC_STREAM_WGS_PER_CU 4 /* capped bulk-streaming grid: workgroups per CU */
 #endif
@@ -1469,6 +1469,35 @@ static int snap_restore(irbc_ctx *c, const Snapshot &S)
     return IRBC_OK;
 }
 
+/* launch options from an index: bit 0 one-workgroup membrane, 1 merged
+ * bounce-back, 2 graphs, 3 XCD chunking, 4 four particles per lane, 5 capped
+ * streaming grid, 6 membrane after this step's streaming, 7 membrane chain,
+ * 8 one queue */
+static void apply_options(irbc_ctx *c, int k)
+{
+    c->mem_fused = (k >> 0) & 1;
+    c->near_merged = (k >> 1) & 1;
+    c->use_graphs = (k >> 2) & 1;
+    c->xcd_chunk = (k >> 3) & 1;
+    c->stream4 = (k >> 4) & 1;
+    c->stream_grid = (k >> 5) & 1;
+    c->mem_after = (k >> 6) & 1;
+    c->mem_chain = (k >> 7) & 1;
+    c->one_queue = (k >> 8) & 1;
+}
+
+/* The indices the autotuner times: the membrane as the chain on one queue,
+ * the chain on its own stream, or the five grid kernels on its own stream,
+ * times bounce-back shape, graphs, XCD chunking and stream shape (48).  The
+ * one-workgroup membrane, the capped streaming grid and the late membrane
+ * start never won and are left to irbc_set_option. */
+static bool tune_candidate(int k)
+{
+    if (k & ((1 << 0) | (1 << 5) | (1 << 6))) return false;
+    const bool chain = (k >> 7) & 1, oneq = (k >> 8) & 1;
+    return chain || !oneq;
+}
+
 int irbc_autotune(irbc_ctx *c, int32_t steps, double *report)
 {
     if (!c || steps < 1) return fail(IRBC_ERR_ARG, "irbc_autotune: bad arguments");
@@ -1481,20 +1510,11 @@ int irbc_autotune(irbc_ctx *c, int32_t steps, double *report)
     double best = 1e300;
     int best_k = 0;
     for (int k = 0; k < IRBC_TUNE_CANDIDATES; ++k) {
-        if (((k >> 7) & 1) && (k & 1)) {
-            /* the chain takes precedence over the one-workgroup membrane:
-             * the same launches as k - 1 */
+        if (!tune_candidate(k)) {
             if (report) report[k] = std::nan("");
             continue;
         }
-        c->mem_chain = (k >> 7) & 1;
-        c->mem_fused = (k >> 0) & 1;
-        c->near_merged = (k >> 1) & 1;
-        c->use_graphs = (k >> 2) & 1;
-        c->xcd_chunk = (k >> 3) & 1;
-        c->stream4 = (k >> 4) & 1;
-        c->stream_grid = (k >> 5) & 1;
-        c->mem_after = (k >> 6) & 1;
+        apply_options(c, k);
         if (!rc) rc = snap_restore(c, S);
         if (!rc) rc = irbc_md_steps(c, steps); /* warm-up; also captures graphs */
         double ms = 0.0;
@@ -1518,14 +1538,7 @@ int irbc_autotune(irbc_ctx *c, int32_t steps, double *report)
             best_k = k;
         }
     }
-    c->mem_chain = (best_k >> 7) & 1;
-    c->mem_fused = (best_k >> 0) & 1;
-    c->near_merged = (best_k >> 1) & 1;
-    c->use_graphs = (best_k >> 2) & 1;
-    c->xcd_chunk = (best_k >> 3) & 1;
-    c->stream4 = (best_k >> 4) & 1;
-    c->stream_grid = (best_k >> 5) & 1;
-    c->mem_after = (best_k >> 6) & 1;
+    apply_options(c, best_k);
     if (!rc && c->use_graphs) {
         /* capture the chosen shape's graphs for one whole collide/list-update
          * cycle now, so that later runs replay instead of capturing */

@@ -41,7 +41,7 @@ OPT_CHAIN_DEVICE_SCOPE = 9
 OPT_NEAR_MASKS = 10
 OPT_ONE_QUEUE = 11
 OPTIONS = (("membrane_fused", 1), ("near_merged", 3), ("graphs", 2), ("xcd_chunk", 4), ("stream4", 5),
-           ("stream_grid", 6), ("mem_after_stream", 7), ("membrane_chain", 8))
+           ("stream_grid", 6), ("mem_after_stream", 7), ("membrane_chain", 8), ("one_queue", 11))
 
 SERIES_FIELDS = ("step", "x", "y", "z", "vx", "vy", "vz", "bonds")
 
@@ -407,9 +407,9 @@ class Batch:
 
     def autotune(self, steps=20):
         """Pick the fastest launch options for this batch; returns
-        (chosen index, ms per step of each of the 256 candidates, NaN where
-        not run); bit i of the index is option OPTIONS[i]."""
-        rep = np.zeros(256)
+        (chosen index, ms per step of each of the 512 indices, NaN where
+        not timed); bit i of the index is option OPTIONS[i]."""
+        rep = np.zeros(512)
         k = _chk(self.L.irbc_autotune(self.h, int(steps), _ptr(rep, _d)))
         return k, rep
 

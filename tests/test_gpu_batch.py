@@ -204,8 +204,11 @@ def test_autotune_leaves_the_trajectory_alone():
     k, rep = b1.autotune(12)
     assert 0 <= k < 2 ** len(ia.OPTIONS) == len(rep)
     idx = np.arange(len(rep))
-    skipped = ((idx >> 7) & 1) & (idx & 1) == 1  # the chain overrides the one-workgroup membrane
-    assert np.all(np.isnan(rep[skipped])) and np.all(rep[~skipped] > 0) and not skipped[k]
+    # timed: no one-workgroup membrane, capped grid or late membrane start;
+    # one queue only with the chain
+    timed = ((idx & 0b1100001) == 0) & (((idx >> 7) & 1) | (1 - ((idx >> 8) & 1)) == 1)
+    assert timed.sum() == 48
+    assert np.all(np.isnan(rep[~timed])) and np.all(rep[timed] > 0) and timed[k]
     b1.md_steps(25)
     b1.sync()
     b2 = en.batch(ds)
