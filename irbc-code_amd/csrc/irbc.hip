@@ -210,7 +210,7 @@ struct irbc_ctx {
     int near_masks = 1;         /* sub-cell near masks (1) or every particle of a listed cell queued (0) */
     int one_queue = 1;          /* with the chain: the step on one queue, the membrane inside the solvent's launches */
     int ncu = 256;              /* compute units of the device */
-    std::map<uint64_t, hipGraphExec_t> graphs;
+    std::map<std::pair<uint64_t, uint64_t>, hipGraphExec_t> graphs;
     int nrep = 0;
     std::vector<HostRep> hr;
     std::vector<DevRep> rep;
@@ -1990,13 +1990,15 @@ static void next_flags(irbc_ctx *c, std::vector<HostRep> &hr, bool &col, bool &l
  * each step, the starting parity and the launch options. */
 static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
 {
-    const uint64_t key = flags | ((uint64_t)(c->par & 1) << 48) | ((uint64_t)n << 50) |
-                         ((uint64_t)(c->mem_fused & 1) << 58) | ((uint64_t)(c->diag_on & 1) << 59) |
-                         ((uint64_t)(c->near_merged & 1) << 60) | ((uint64_t)(c->xcd_chunk & 1) << 61) |
-                         ((uint64_t)(c->stream4 & 1) << 62) | ((uint64_t)(c->stream_grid & 1) << 63) |
-                         ((uint64_t)(c->mem_after & 1) << 54) | ((uint64_t)(c->mem_chain & 1) << 55) | ((uint64_t)(c->chain_dev & 1) << 56) |
-                         ((uint64_t)(c->near_masks & 1) << 57) |
-                         ((uint64_t)(c->one_queue & 1) << 53);
+    /* first: 2 flag bits per step; second: parity, chunk length and every
+     * option that changes the launches, each in a field of its own */
+    const uint64_t shape = (uint64_t)(c->par & 1) | ((uint64_t)n << 1) | ((uint64_t)(c->mem_fused & 1) << 8) |
+                           ((uint64_t)(c->diag_on & 1) << 9) | ((uint64_t)(c->near_merged & 1) << 10) |
+                           ((uint64_t)(c->xcd_chunk & 1) << 11) | ((uint64_t)(c->stream4 & 1) << 12) |
+                           ((uint64_t)(c->stream_grid & 1) << 13) | ((uint64_t)(c->mem_after & 1) << 14) |
+                           ((uint64_t)(c->mem_chain & 1) << 15) | ((uint64_t)(c->chain_dev & 1) << 16) |
+                           ((uint64_t)(c->near_masks & 1) << 17) | ((uint64_t)(c->one_queue & 1) << 18);
+    const std::pair<uint64_t, uint64_t> key(flags, shape);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         const int par0 = c->par;
