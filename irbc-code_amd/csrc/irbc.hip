@@ -1135,14 +1135,21 @@ static int launch_cell_mask(irbc_ctx *c, int staged)
     return IRBC_OK;
 }
 
-static unsigned near_blocks(const irbc_ctx *c)
+/* Bounce-back grids per replica.  The queue holds the particles of marked
+ * sub-cells, 0.2-1% of a replica (up to ~2% while the drift margin is
+ * large), and every queued particle should find its lanes in one pass:
+ * the one-pass kernel gives a particle 16 lanes (16 particles per block),
+ * the sphere pass one lane (256 per block). */
+static int64_t max_np(const irbc_ctx *c)
 {
     int64_t maxnp = 0;
     for (const DevRep &R : c->rep) maxnp = std::max<int64_t>(maxnp, R.np);
-    /* the queue holds the particles of marked sub-cells, about 0.2-1% of a
-     * replica: a quarter of a percent per pass */
-    return (unsigned)std::max<int64_t>(16, (maxnp + 4095) / 4096);
+    return maxnp;
 }
+
+static unsigned near_blocks(const irbc_ctx *c) { return (unsigned)std::max<int64_t>(16, (max_np(c) + 2047) / 2048); }
+
+static unsigned sphere_blocks(const irbc_ctx *c) { return (unsigned)std::max<int64_t>(8, (max_np(c) + 12799) / 12800); }
 
 static int launch_near(irbc_ctx *c)
 {
@@ -1154,7 +1161,7 @@ static int launch_near(irbc_ctx *c)
         LAUNCHCHK();
         return IRBC_OK;
     }
-    k_near_sphere<<<dim3(std::max(4u, near_blocks(c) / 8u), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
+    k_near_sphere<<<dim3(sphere_blocks(c), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
         c->d_rep, c->lloff, c->llent, c->nearq, c->deferq, tk, ts, c->par);
     k_near_hit<<<dim3(std::max(16u, near_blocks(c) / 8u), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
         c->d_rep, c->lloff, c->llent, c->deferq, tk, ts, c->par);
@@ -1911,7 +1918,7 @@ static int launch_step_near(irbc_ctx *c, int phases)
         k_step_near<true><<<dim3(near_blocks(c) + 1, nr), IRBC_WG, 0, c->st>>>(c->d_rep, c->lloff, c->llent,
                                                                            c->nearq, tk, ts, c->par, phases, Mm);
     } else {
-        k_near_sphere<<<dim3(std::max(4u, near_blocks(c) / 8u), nr), IRBC_WG, 0, c->st>>>(
+        k_near_sphere<<<dim3(sphere_blocks(c), nr), IRBC_WG, 0, c->st>>>(
             c->d_rep, c->lloff, c->llent, c->nearq, c->deferq, tk, ts, c->par);
         k_step_near<false><<<dim3(std::max(16u, near_blocks(c) / 8u) + 1, nr), IRBC_WG, 0, c->st>>>(
             c->d_rep, c->lloff, c->llent, c->deferq, tk, ts, c->par, phases, Mm);
