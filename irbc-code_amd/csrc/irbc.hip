@@ -91,6 +91,11 @@ int32_t lig_count(double L, double sp)
 
 /* upper bound of vertex incidences: 2 per edge end, 3 per triangle corner, 1 per hinge corner */
 int32_t inc_bound(const irbc_replica_spec &s) { return 4 * s.n_edges + 9 * s.n_triangles + 4 * s.n_edges; }
+/* Element terms are stored vertex-interleaved: term k of vertex v at
+ * k * nv + v, so that the lanes of a wave, one vertex each, read adjacent
+ * terms together.  The planes run to the largest per-vertex count, held to
+ * twice the mean (a valence up to about twice the mean valence). */
+int64_t scr_bound(const irbc_replica_spec &s) { return 2 * (int64_t)inc_bound(s); }
 
 Sizes plan(const irbc_replica_spec *specs, int n, std::vector<DevRep> *reps,
            std::vector<ScanBlk> *blks, std::vector<int32_t> *rep_first_blk,
@@ -160,7 +165,7 @@ Sizes plan(const irbc_replica_spec *specs, int n, std::vector<DevRep> *reps,
         z.nligx += R.nligx;
         z.nligy += R.nligy;
         z.llent += R.llent_cap;
-        z.scr += 8 * (int64_t)s.n_edges + 9 * (int64_t)s.n_triangles;
+        z.scr += scr_bound(s);
         z.inc += inc_bound(s);
         z.part += (std::max(s.n_edges, s.n_triangles) + IRBC_WG - 1) / IRBC_WG;
         z.maxnv = std::max(z.maxnv, s.n_vertices);
@@ -716,9 +721,15 @@ int irbc_set_mesh(irbc_ctx *c, int r, const double *xyz, const int32_t *edges, c
         order.insert(order.end(), inc[v].begin(), inc[v].end());
     }
     if ((int64_t)order.size() != inc_bound(h.spec)) return fail(IRBC_ERR_ARG, "irbc_set_mesh: every edge needs two hinge partners");
-    /* element term -> position in the vertex-major order */
+    /* element term -> its slot: term k of vertex v (in the reference's
+     * summation order) at k * nv + v */
+    size_t kmax = 0;
+    for (int v = 0; v < nv; ++v) kmax = std::max(kmax, inc[v].size());
+    if ((int64_t)kmax * nv > scr_bound(h.spec))
+        return fail(IRBC_ERR_ARG, "irbc_set_mesh: a vertex has %zu element terms, more than twice the mean", kmax);
     std::vector<int32_t> ient(order.size());
-    for (size_t i = 0; i < order.size(); ++i) ient[order[i]] = (int32_t)i;
+    for (int v = 0; v < nv; ++v)
+        for (size_t k = 0; k < inc[v].size(); ++k) ient[inc[v][k]] = (int32_t)(k * nv + v);
     R.Sarea0 = A0;
     R.Volume0 = V0;
     R.theta0 = theta0;
@@ -1241,9 +1252,9 @@ static int membrane_launches(const irbc_ctx *c) { return (c->mem_chain || c->mem
 static unsigned chain_blocks(const irbc_ctx *c)
 {
     const unsigned vb = (unsigned)((c->z.maxnv + IRBC_WG - 1) / IRBC_WG);
-    const unsigned eb = (unsigned)((std::max(c->z.maxne, c->z.maxnt) + IRBC_WG - 1) / IRBC_WG);
+    const unsigned eb = (unsigned)((c->z.maxne + IRBC_WG - 1) / IRBC_WG);
     const unsigned tb = (unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG);
-    return vb + (eb + vb) + tb + vb + 1;
+    return vb + (eb + tb + vb) + tb + vb + 1;
 }
 
 static int membrane_launch(irbc_ctx *c, int phases, hipStream_t s, int k)
@@ -2081,6 +2092,18 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
     }
     return join_chains(c);
 }
+
+#ifdef IRBC_CHAIN_CLOCK
+/* diagnostic builds only: the chain's clock records (3 per ticket, 64
+ * tickets) of the last launch */
+extern "C" int irbc_debug_chain_clock(irbc_ctx *c, long long *out)
+{
+    if (!c || !out) return fail(IRBC_ERR_ARG, "irbc_debug_chain_clock: bad arguments");
+    HIPCHK(hipDeviceSynchronize());
+    HIPCHK(hipMemcpyFromSymbol(out, HIP_SYMBOL(chain_clock_buf), sizeof(long long) * 3 * 256));
+    return IRBC_OK;
+}
+#endif
 
 int irbc_set_option(irbc_ctx *c, int option, int value)
 {

@@ -64,7 +64,7 @@ struct DevRep {
     int64_t llent_cap;
     int32_t inc_off;   /* vertex-major incidence offsets at voff + r */
     int32_t inc_ent_off; /* element term -> its position in the vertex-major order */
-    int64_t scr_off;   /* element terms (double3), stored in vertex-major order */
+    int64_t scr_off;   /* element terms (double3): term k of vertex v at k * nv + v */
     int32_t rep_index;
     int32_t part_off;  /* first per-block area/volume partial of this replica */
     /* ---- parameters ---- */
