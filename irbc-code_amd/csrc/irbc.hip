@@ -188,7 +188,7 @@ int64_t bytes_of(const Sizes &z, int n)
     b += (int64_t)z.nv * (31 * 8 + 12 * 4 + IRBC_MAX_CAND * (4 + 8)); /* incl. vertex kinematics x2 */
     b += 2 * (int64_t)z.nt * (IRBC_TRI_KIN * 8 + 16) + (int64_t)z.part * 16 + z.slots * 8;
     b += (int64_t)z.ne * (5 * 4 + 2 * 8) + (int64_t)z.nt * (3 * 4 + 2 * 8);
-    b += z.scr * 24 + (int64_t)z.inc * 4 + (int64_t)(z.nv + n) * 4;
+    b += z.scr * 32 + (int64_t)z.inc * 4 + (int64_t)(z.nv + n) * 4;
     b += (int64_t)z.nlig * 4 + (int64_t)(z.nligx + z.nligy) * 8;
     b += (int64_t)z.scanblk * 4 + (int64_t)n * sizeof(DevRep);
     return b;
@@ -243,7 +243,7 @@ struct irbc_ctx {
     double *mx, *my, *mz, *mvx, *mvy, *mvz, *mfx, *mfy, *mfz, *poff, *pon;
     int32_t *mix, *miy, *mpid, *bondi, *bondl, *knob, *vnext;
     int32_t *e1, *e2, *es1, *es2, *iedge, *t1, *t2, *t3;
-    double *l0, *kappap, *tarea, *tri_area, *wca, *slot, *upos, *part, *tri_kin, *vkin;
+    double *l0, *kappap, *tarea, *tri_area, *wca, *slot, *tag, *upos, *part, *tri_kin, *vkin;
     float4 *tri_sph;
     int32_t *inc_off, *inc_ent;
     int32_t *ncand, *cand, *draw_off, *flag, *list;
@@ -289,7 +289,7 @@ struct irbc_ctx {
         M.e1 = e1; M.e2 = e2; M.es1 = es1; M.es2 = es2; M.iedge = iedge;
         M.t1 = t1; M.t2 = t2; M.t3 = t3;
         M.l0 = l0; M.kappap = kappap; M.tarea = tarea;
-        M.tri_area = tri_area; M.wca = wca; M.slot = slot;
+        M.tri_area = tri_area; M.wca = wca; M.slot = slot; M.tag = tag;
         M.upos = upos; M.upos_base = 0; M.part = part; M.tri_kin = tri_kin + (tbuf & 1) * tri_stride();
         M.tri_sph = tri_sph + (tbuf & 1) * z.nt;
         M.vkin = vkin + (tbuf & 1) * 6 * (int64_t)z.nv;
@@ -467,7 +467,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     A(c->e1, z.ne) A(c->e2, z.ne) A(c->es1, z.ne) A(c->es2, z.ne) A(c->iedge, z.ne)
     A(c->t1, z.nt) A(c->t2, z.nt) A(c->t3, z.nt)
     A(c->l0, z.ne) A(c->kappap, z.ne) A(c->tarea, z.nt) A(c->tri_area, z.nt) A(c->wca, 3 * z.nv)
-    A(c->slot, 3 * z.scr) A(c->inc_off, z.nv + n) A(c->inc_ent, z.inc)
+    A(c->slot, 3 * z.scr) A(c->tag, z.scr) A(c->inc_off, z.nv + n) A(c->inc_ent, z.inc)
     A(c->upos, 4 * (int64_t)z.nv) A(c->part, 2 * (int64_t)z.part) A(c->tri_kin, 2 * (int64_t)IRBC_TRI_KIN * z.nt) A(c->tri_sph, 2 * (int64_t)z.nt) A(c->vkin, 12 * (int64_t)z.nv)
     A(c->ncand, z.nv) A(c->cand, (int64_t)z.nv * IRBC_MAX_CAND) A(c->draw_off, z.nv)
     A(c->flag, z.nv) A(c->list, z.nv)
@@ -759,6 +759,9 @@ int irbc_set_mesh(irbc_ctx *c, int r, const double *xyz, const int32_t *edges, c
     HIPCHK(hipMemcpyAsync(c->tarea + R.toff, ta.data(), nt * 8, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->inc_off + R.inc_off, ioff.data(), (nv + 1) * 4, hipMemcpyHostToDevice, s));
     HIPCHK(hipMemcpyAsync(c->inc_ent + R.inc_ent_off, ient.data(), ient.size() * 4, hipMemcpyHostToDevice, s));
+    /* every slot's tag NaN (all bytes 0xFF): a finished term, until the
+     * triangle terms tag theirs */
+    HIPCHK(hipMemsetAsync(c->tag + R.scr_off, 0xFF, (size_t)scr_bound(h.spec) * 8, s));
     h.mesh = true;
     return c->push(r);
 }
@@ -1254,7 +1257,7 @@ static unsigned chain_blocks(const irbc_ctx *c)
     const unsigned vb = (unsigned)((c->z.maxnv + IRBC_WG - 1) / IRBC_WG);
     const unsigned eb = (unsigned)((c->z.maxne + IRBC_WG - 1) / IRBC_WG);
     const unsigned tb = (unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG);
-    return vb + (eb + tb + vb) + tb + vb + 1;
+    return vb + (eb + tb + vb) + vb + 1;
 }
 
 static int membrane_launch(irbc_ctx *c, int phases, hipStream_t s, int k)
@@ -2100,7 +2103,7 @@ extern "C" int irbc_debug_chain_clock(irbc_ctx *c, long long *out)
 {
     if (!c || !out) return fail(IRBC_ERR_ARG, "irbc_debug_chain_clock: bad arguments");
     HIPCHK(hipDeviceSynchronize());
-    HIPCHK(hipMemcpyFromSymbol(out, HIP_SYMBOL(chain_clock_buf), sizeof(long long) * 3 * 256));
+    HIPCHK(hipMemcpyFromSymbol(out, HIP_SYMBOL(chain_clock_buf), sizeof(long long) * 6 * 256));
     return IRBC_OK;
 }
 #endif
