@@ -1257,7 +1257,9 @@ static unsigned chain_blocks(const irbc_ctx *c)
     const unsigned vb = (unsigned)((c->z.maxnv + IRBC_WG - 1) / IRBC_WG);
     const unsigned eb = (unsigned)((c->z.maxne + IRBC_WG - 1) / IRBC_WG);
     const unsigned tb = (unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG);
-    return vb + (eb + tb + vb) + vb + 1;
+    const unsigned VPB = IRBC_WG / IRBC_GATHER_LANES; /* gather: vertices per workgroup */
+    const unsigned gb = (unsigned)((c->z.maxnv + VPB - 1) / VPB);
+    return vb + (eb + tb + vb) + gb + 1;
 }
 
 static int membrane_launch(irbc_ctx *c, int phases, hipStream_t s, int k)

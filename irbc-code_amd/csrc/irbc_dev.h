@@ -40,6 +40,14 @@
 #define IRBC_RS_BOND 5u
 
 #define IRBC_SERIES_REC 8      /* doubles per time-series record */
+/* lanes per vertex in the membrane chain's gather.  More lanes fetch a
+ * vertex's terms in fewer round trips but add workgroups that hold wave
+ * slots while they wait; measured on the 64x64x48 replica (one queue / two
+ * queues with graphs / 64-replica batch, us per step): 1: 59.0 / 61.8 /
+ * 378.5, 2: 59.9 / 60.6 / 382.2, 4: 62.1 / 59.1 / 391.5. */
+#ifndef IRBC_GATHER_LANES
+#define IRBC_GATHER_LANES 1
+#endif
 
 /* vertex drift up to which the cells' near masks stand (see k_cell_mask) */
 #define IRBC_DRIFT_MAX 0.25
