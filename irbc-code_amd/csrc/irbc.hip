@@ -1163,7 +1163,13 @@ static int64_t max_np(const irbc_ctx *c)
 
 static unsigned near_blocks(const irbc_ctx *c) { return (unsigned)std::max<int64_t>(16, (max_np(c) + 2047) / 2048); }
 
-static unsigned sphere_blocks(const irbc_ctx *c) { return (unsigned)std::max<int64_t>(8, (max_np(c) + 12799) / 12800); }
+/* the sphere pass: IRBC_SPHERE_LANES lanes per queued particle, sized for a
+ * queue of up to 2% of a replica's particles in one pass */
+static unsigned sphere_blocks(const irbc_ctx *c)
+{
+    const int64_t per_block = IRBC_WG / IRBC_SPHERE_LANES;
+    return (unsigned)std::max<int64_t>(8, (max_np(c) / 50 + per_block - 1) / per_block);
+}
 
 static int launch_near(irbc_ctx *c)
 {
