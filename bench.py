@@ -176,7 +176,7 @@ def served(n, rate, steps, budget_gb, plain_n=3):
     """Poisson arrivals of uneven-ensemble requests through the request
     loop, against the same requests run one at a time."""
     reqs = serve.poisson_requests(n, rate, steps, seed=1)
-    srv = serve.Server(int(budget_gb * 2 ** 30), chunk=250)
+    srv = serve.Server(int(budget_gb * 2 ** 30), chunk=250, max_active=4, window=0.1)
     srv.run(reqs)
     out = serve.summary(reqs, srv.makespan)
     t_plain, s_plain = 0.0, 0
@@ -190,7 +190,7 @@ def served(n, rate, steps, budget_gb, plain_n=3):
         a.close()
     out.update({"workload": "%d requests, Poisson arrivals at %.1f/s, %d steps each, boxes 24^3..64x64x48 "
                             "at density 5-10, 162/642/2562-vertex ellipsoids" % (n, rate, steps),
-                "hbm_budget_GiB": budget_gb, "chunk_steps": srv.chunk,
+                "hbm_budget_GiB": budget_gb, "chunk_steps": srv.chunk, "batching_window_s": srv.window,
                 "micro_batches": srv.stats["groups"], "max_concurrent_micro_batches": srv.stats["max_active_groups"],
                 "max_resident_replicas": srv.stats["max_resident_replicas"],
                 "plain_replica_steps_per_s": s_plain / t_plain,

@@ -1903,6 +1903,56 @@ static bool one_queue(const irbc_ctx *c) { return c->one_queue && c->mem_chain; 
 /* The membrane chain (phases up to the gather) of this step, this step's
  * triangle kinematics (tri_kin) and its bulk streaming in one launch;
  * e0/e1 (both or neither) are events bound to the dispatch. */
+#ifndef IRBC_CHAIN_RESIDENCY
+#define IRBC_CHAIN_RESIDENCY 1.0 /* share of the launch a chain workgroup holds its wave slots */
+#endif
+/* The bulk units' split over the XCDs for the fused streaming launch, a
+ * contiguous range per XCD.  With fewer replicas than XCDs some XCDs host
+ * no chain, and XCD x's weight is 1 - IRBC_CHAIN_RESIDENCY x (the share of
+ * its wave slots its replicas' chain workgroups hold; replica r's chain runs
+ * on XCD r % 8).  With more, every XCD hosts chains and the shares are
+ * equal.  Measured (us per step, one box): the 64x64x48 replica 59.3 with
+ * the tiles dealt round-robin, 58.8 with equal ranges, 56.8 weighted; a
+ * mixed 16-request group 293 round-robin, 257 equal, 307 weighted; the
+ * 64-replica batch unchanged. */
+static XcdSplit xcd_split(const irbc_ctx *c, int units)
+{
+    const double res = c->nrep < IRBC_NXCD ? IRBC_CHAIN_RESIDENCY : 0.0;
+    const double slots = 32.0 * 4.0 * 4.0 / 4.0; /* 32 CUs x 4 SIMDs x 4 waves, in 4-wave workgroups */
+    double w[IRBC_NXCD], ws = 0.0;
+    for (int x = 0; x < IRBC_NXCD; ++x) w[x] = 0.0;
+    for (int r = 0; r < c->nrep; ++r) {
+        const DevRep &R = c->rep[r];
+        const int vb = (R.nv + IRBC_WG - 1) / IRBC_WG, eb = (R.ne + IRBC_WG - 1) / IRBC_WG,
+                  tb = (R.nt + IRBC_WG - 1) / IRBC_WG;
+        const int VPB = IRBC_WG / IRBC_GATHER_LANES;
+        w[r % IRBC_NXCD] += vb + (eb + tb + vb) + (R.nv + VPB - 1) / VPB; /* the chain's workgroups */
+    }
+    for (int x = 0; x < IRBC_NXCD; ++x) {
+        w[x] = 1.0 - res * std::min(0.9, w[x] / slots);
+        ws += w[x];
+    }
+    XcdSplit xs;
+    int given = 0;
+    for (int x = 0; x < IRBC_NXCD; ++x) {
+        xs.q[x] = (int)std::floor(units * w[x] / ws);
+        given += xs.q[x];
+    }
+    /* the rounding remainder to the XCDs with the most weight */
+    for (int k = 0; given < units; ++k, ++given) {
+        int best = 0;
+        for (int x = 1; x < IRBC_NXCD; ++x)
+            if (w[x] - (double)xs.q[x] / units * ws > w[best] - (double)xs.q[best] / units * ws) best = x;
+        xs.q[best] += 1;
+    }
+    int o = 0;
+    for (int x = 0; x < IRBC_NXCD; ++x) {
+        xs.o[x] = o;
+        o += xs.q[x];
+    }
+    return xs;
+}
+
 static int launch_step_stream(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1, bool tri_kin, int phases)
 {
     const MemDev Mm = c->mem(c->par ^ 1); /* the membrane writes the next step's parity */
@@ -1912,19 +1962,24 @@ static int launch_step_stream(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1, bool tr
     const int CB = IRBC_NXCD * NB * ((c->nrep + IRBC_NXCD - 1) / IRBC_NXCD);
     const int tkb = std::max(1, (c->z.maxnt + IRBC_WG - 1) / IRBC_WG);
     const int ntk = tri_kin ? tkb * c->nrep : 0;
+    const int ntkp = IRBC_NXCD * ((ntk + IRBC_NXCD - 1) / IRBC_NXCD); /* the bulk starts on XCD 0 */
     const int T = (int)c->z.tiles;
     const int32_t *tr = c->d_tile_rep, *sk = c->skey2, *lo = c->lloff;
     const double *ct = c->ctab;
     const uint64_t *cm = c->cmask;
     const bool four = c->stream4 != 0;
-    const unsigned G = (unsigned)(CB + ntk + (four ? (T + IRBC_WG / 64 - 1) / (IRBC_WG / 64) : T));
+    const int units = four ? (T + IRBC_WG / 64 - 1) / (IRBC_WG / 64) : T;
+    const XcdSplit xs = xcd_split(c, units);
+    int qmax = 0;
+    for (int x = 0; x < IRBC_NXCD; ++x) qmax = std::max(qmax, xs.q[x]);
+    const unsigned G = (unsigned)(CB + ntkp + IRBC_NXCD * qmax);
     auto *kern = four ? k_step_stream<true> : k_step_stream<false>;
     if (e0)
         hipExtLaunchKernelGGL(kern, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep, c->nrep, phases, Mm, NB,
-                              CB, tr, T, sk, ct, lo, cm, c->nearq, c->par, tk, tkb, ntk);
+                              CB, tr, T, sk, ct, lo, cm, c->nearq, c->par, tk, tkb, ntk, ntkp, xs);
     else
         kern<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, c->nrep, phases, Mm, NB, CB, tr, T, sk, ct, lo, cm, c->nearq, c->par,
-                                       tk, tkb, ntk);
+                                       tk, tkb, ntk, ntkp, xs);
     LAUNCHCHK();
     return IRBC_OK;
 }

@@ -13,7 +13,11 @@ The loop
     arrival order, while their device bytes (irbc_estimate_bytes) fit the
     HBM budget left, into one context (a micro-batch: one launch sequence
     for all of them), so requests that arrive while the GPU is busy are
-    batched together;
+    batched together.  With a batching `window` > 0 it forms a micro-batch
+    only once the oldest waiting request has waited that long (or a full
+    group is waiting): a few hundred milliseconds of waiting turn a trickle
+    of one-replica micro-batches into a few large ones, which is where the
+    GPU's throughput is;
   * keeps every admitted replica's state on the device until it is done;
   * advances every active context by `chunk` steps at a time, each on its
     own pair of HIP streams, so micro-batches admitted at different times
@@ -65,6 +69,10 @@ class _Group:
             if r.series_every > 0:
                 self.batch.set_series(i, r.series_every, max(1, r.steps // r.series_every))
         en.start(self.batch)
+        if len(reqs) >= 8:
+            # a large micro-batch streams four particles per lane, as the
+            # autotuner picks for the 64-replica batch
+            self.batch.set_option(ia.OPT_STREAM4, 1)
         self.steps = reqs[0].steps
         self.done_steps = 0
         self.bytes = self.batch.device_bytes()
@@ -73,11 +81,12 @@ class _Group:
 
 
 class Server:
-    def __init__(self, budget_bytes, chunk=250, max_group=64, max_active=2):
+    def __init__(self, budget_bytes, chunk=250, max_group=64, max_active=2, window=0.0):
         self.budget = int(budget_bytes)
         self.chunk = int(chunk)
         self.max_group = int(max_group)
         self.max_active = int(max_active)
+        self.window = float(window)
         self.used = 0
         self.active = []
         self.stats = {"groups": 0, "max_active_groups": 0, "max_resident_replicas": 0}
@@ -88,6 +97,14 @@ class Server:
         together); a request that does not fit waits for memory."""
         while queue and queue[0].arrival <= now and len(self.active) < self.max_active:
             head = queue[0]
+            if now - head.arrival < self.window:
+                arrived = 0
+                for r in queue:
+                    if r.arrival > now or arrived >= self.max_group:
+                        break
+                    arrived += 1
+                if arrived < self.max_group:
+                    return  # wait for more arrivals to batch with
             if head.bytes > self.budget:
                 raise ValueError("request %s needs %d bytes, more than the whole budget %d"
                                  % (head.id, head.bytes, self.budget))
