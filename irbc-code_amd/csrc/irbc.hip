@@ -1084,14 +1084,15 @@ static int launch_collision(irbc_ctx *c, int staged, int diag)
     const unsigned T = (unsigned)c->z.tiles, CT = (unsigned)c->z.ctiles;
     if (staged) k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 0);
     const int xc = c->xcd_chunk;
-    k_bin<<<xc ? xcd_grid(T) : T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, xc, staged, c->skey,
-                                                       c->srank, c->ccount);
+    MemDev M = c->mem(c->par);
+    const unsigned nvb = IRBC_NXCD * ((unsigned)(c->nrep + IRBC_NXCD - 1) / IRBC_NXCD);
+    k_bin<<<nvb + (xc ? xcd_grid(T) : T), IRBC_WG, 0, c->st>>>(c->d_rep, c->nrep, (int)nvb, c->d_tile_rep, (int)T, xc,
+                                                              staged, c->skey, c->srank, c->ccount, M, c->vhead,
+                                                              c->vnext);
     int rc = launch_scan(c, staged, 0, c->ccount, c->cstart);
     if (rc) return rc;
     k_scatter<<<xc ? xcd_grid(T) : T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, xc, staged, c->skey,
                                                            c->srank, c->cstart, c->skey2);
-    MemDev M = c->mem(c->par);
-    k_vbin<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, staged, M, c->vhead, c->vnext);
     k_cell<<<xc ? xcd_grid(CT) : CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, xc, staged, c->cstart,
                                                          c->vhead,
                                                 c->vnext,

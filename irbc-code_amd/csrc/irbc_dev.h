@@ -209,12 +209,14 @@ __device__ __forceinline__ int32_t cell_of(int fx, int fy, int fz, int Lx, int L
  * Blocks with xcd_tile(n) >= n do nothing.  With chunked = 0 the block
  * index is the tile (a grid of exactly n blocks). */
 #define IRBC_NXCD 8
-__device__ __forceinline__ int xcd_tile(int n, int chunked)
+__device__ __forceinline__ int xcd_tile_of(int n, int chunked, int b)
 {
-    if (!chunked) return (int)blockIdx.x;
+    if (!chunked) return b;
     const int per = (n + IRBC_NXCD - 1) / IRBC_NXCD;
-    return (int)(blockIdx.x % IRBC_NXCD) * per + (int)(blockIdx.x / IRBC_NXCD);
+    return (b % IRBC_NXCD) * per + b / IRBC_NXCD;
 }
+
+__device__ __forceinline__ int xcd_tile(int n, int chunked) { return xcd_tile_of(n, chunked, (int)blockIdx.x); }
 
 __device__ __forceinline__ void cross3(const double a[3], const double b[3], double c[3])
 {
