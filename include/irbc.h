@@ -157,8 +157,9 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
  *   as HIP graphs (one per chunk shape) and replays them; 0: launches
  *   kernel by kernel.  Ignored while event profiling is on. */
 #define IRBC_OPT_GRAPHS 2
-/* IRBC_OPT_NEAR_MERGED  1: bounce-back off the membrane in one kernel;
- *   0 (default): a bounding-sphere pass, one lane per particle, then a hit
+/* IRBC_OPT_NEAR_MERGED  1 (default for fewer than 8 replicas): bounce-back
+ *   off the membrane in one kernel, 16 lanes per particle; 0 (default for 8
+ *   or more): a bounding-sphere pass, four lanes per particle, then a hit
  *   pass over the particles that can reach a triangle.  Same results. */
 #define IRBC_OPT_NEAR_MERGED 3
 /* IRBC_OPT_XCD_CHUNK  1 (default): the binning, scatter and cell kernels

@@ -205,7 +205,7 @@ struct irbc_ctx {
     int mem_fused = 0;          /* membrane step as one workgroup per replica (1) or five grid kernels (0) */
     int mem_lds_nv = 0;         /* largest mesh whose unwrapped positions the one-workgroup step keeps in LDS */
     int use_graphs = 1;         /* irbc_md_steps replays captured chunks of IRBC_GRAPH_STEPS steps */
-    int near_merged = 0;        /* near-membrane streaming in one kernel (1) or sphere pass + hit pass (0) */
+    int near_merged = 0;        /* near-membrane streaming in one kernel (1) or sphere pass + hit pass (0); set by irbc_create */
     int xcd_chunk = 1;          /* sort and cell kernels hand each XCD a contiguous eighth of the tiles */
     int stream4 = 0;            /* bulk streaming with four particles per lane */
     int stream_grid = 0;        /* bulk streaming on a capped grid (IRBC_STREAM_WGS_PER_CU per CU) */
@@ -421,6 +421,10 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     irbc_ctx *c = new irbc_ctx();
     c->device = device;
     c->nrep = n;
+    /* the one-pass bounce-back suits a few replicas (the single 64x64x48
+     * replica: 60 against 71 us per step), the two passes a batch (64
+     * replicas: 359 against 439) */
+    c->near_merged = n < IRBC_NXCD ? 1 : 0;
     {
         /* dynamic LDS of the one-workgroup membrane step: 32 bytes a vertex,
          * within 64 KiB per workgroup less 16 KiB for its static arrays */
