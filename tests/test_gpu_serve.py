@@ -50,6 +50,31 @@ def test_served_results_equal_alone_and_admission_by_budget():
 
 
 @pytest.mark.gpu
+def test_batching_window_groups_arrivals_and_keeps_results():
+    """With a batching window the loop waits for arrivals and forms fewer,
+    larger micro-batches (eight or more replicas switch to four particles
+    per lane); every result still equals the request served alone."""
+    reps = [r.replica for r in _requests()] * 2
+    reqs = [serve.Request(i, d, 200, 50, 0.002 * i) for i, d in enumerate(reps)]
+    budget = int(4 * sum(r.bytes for r in reqs))
+    srv = serve.Server(budget, chunk=100, max_active=4, window=0.05)
+    srv.run(reqs)
+    assert all(r.finished is not None for r in reqs)
+    assert srv.used == 0
+    assert srv.stats["groups"] == 1  # every request arrived inside the window
+    assert srv.stats["max_resident_replicas"] == len(reqs)
+    # the oldest request waited for the window before it was admitted
+    assert reqs[0].admitted >= 0.05
+    for r in reqs[:5]:
+        alone = serve.Server(r.bytes * 2, chunk=200)
+        a = serve.Request(r.id, r.replica, r.steps, r.series_every, 0.0)
+        alone.run([a])
+        np.testing.assert_array_equal(r.series, a.series, err_msg="request %d series" % r.id)
+        for k in r.final:
+            np.testing.assert_array_equal(r.final[k], a.final[k], err_msg="request %d %s" % (r.id, k))
+
+
+@pytest.mark.gpu
 def test_request_larger_than_budget_is_refused():
     r = _requests()[0]
     srv = serve.Server(r.bytes // 2)
