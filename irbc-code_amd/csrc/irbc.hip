@@ -1249,7 +1249,7 @@ static int launch_tri_kin(irbc_ctx *c, int refresh)
         k_vkin<<<dim3((unsigned)((c->z.maxnv + IRBC_WG - 1) / IRBC_WG), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
             c->d_rep, M);
     k_tri_kin<<<dim3((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
-        c->d_rep, M);
+        c->d_rep, M, c->par & 1);
     LAUNCHCHK();
     return IRBC_OK;
 }

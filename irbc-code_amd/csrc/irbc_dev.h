@@ -118,6 +118,8 @@ struct DevRep {
     int32_t mem_ticket;
     int32_t mem_done[4];
     int32_t nlisted;        /* cells that list a triangle (compacted at each list update) */
+    int32_t mask_keep;      /* the coming mask launch keeps this replica's masks (decided by the
+                             * launch before it, so every mask workgroup sees one decision) */
     /* time series: every `series_every` steps one record of IRBC_SERIES_REC
      * doubles (step, centroid x y z unwrapped, centroid velocity, bonds) */
     int32_t series_every, series_cap, series_n, series_pad;

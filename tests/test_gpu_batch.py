@@ -263,3 +263,29 @@ def test_capped_stream_grid_does_not_change_results():
         for r in range(len(ds)):
             for key in snaps[0][r]:
                 np.testing.assert_array_equal(snaps[0][r][key], other[r][key], err_msg="replica %d %s" % (r, key))
+
+
+@pytest.mark.gpu
+def test_mask_keep_and_queue_are_deterministic():
+    """Two runs of the same batch give the same queue lengths and states,
+    step after step, across collisions that keep or rebuild the near masks.
+    (A rebuild decided inside the mask launch could be seen half-way: its
+    first workgroup clears the drift that the others' decision reads, so
+    some cells kept stale masks and the queues differed between runs.)"""
+    ds = [en.replica(32, 32, 24, 10, nv, None, 1.0 + 0.1 * (r % 4), 100 + 40 * r, 20.0 * (1 + r % 3),
+                     2.0 * (1 + r % 2), 0.05, 5000 + r) for r, nv in enumerate([162, 642, 2562, 642] * 2)]
+    runs = []
+    for _ in range(2):
+        b = en.batch(ds)
+        qs = []
+        for _ in range(12):
+            b.md_steps(25)
+            b.sync()
+            qs.append([b.globals(r)["near_queue"] for r in range(len(ds))])
+            assert all(b.globals(r)["status"] == 0 for r in range(len(ds)))
+        sol = [b.solvent(r)["x"] for r in range(len(ds))]
+        runs.append((np.array(qs), sol))
+        b.close()
+    np.testing.assert_array_equal(runs[0][0], runs[1][0])
+    for r in range(len(ds)):
+        np.testing.assert_array_equal(runs[0][1][r], runs[1][1][r])
