@@ -1097,7 +1097,7 @@ static int launch_collision(irbc_ctx *c, int staged, int diag)
     if (rc) return rc;
     k_scatter<<<xc ? xcd_grid(T) : T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, xc, staged, c->skey,
                                                            c->srank, c->cstart, c->skey2);
-    k_cell<<<xc ? xcd_grid(2 * CT) : 2 * CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, xc, staged, c->cstart,
+    k_cell<<<xc ? xcd_grid(IRBC_CELL_LANES * CT) : IRBC_CELL_LANES * CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, xc, staged, c->cstart,
                                                          c->vhead,
                                                 c->vnext,
                                       c->mvx, c->mvy, c->mvz, c->cell_u, c->wall_z, c->mem_z, c->ctab,
