@@ -44,7 +44,9 @@
  * vertex's terms in fewer round trips but add workgroups that hold wave
  * slots while they wait; measured on the 64x64x48 replica (one queue / two
  * queues with graphs / 64-replica batch, us per step): 1: 59.0 / 61.8 /
- * 378.5, 2: 59.9 / 60.6 / 382.2, 4: 62.1 / 59.1 / 391.5. */
+ * 378.5, 2: 59.9 / 60.6 / 382.2, 4: 62.1 / 59.1 / 391.5; again once the
+ * streaming split weighted the XCDs by their chains (one queue / batch):
+ * 1: 55.4 / 346.3, 2: 55.4 / 350.4, 4: 56.5 / 359.2. */
 #ifndef IRBC_GATHER_LANES
 #define IRBC_GATHER_LANES 1
 #endif
