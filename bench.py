@@ -36,11 +36,13 @@ import irbc_amd as ia  # noqa: E402
 import serve  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per particle per streaming launch: x,y,z,vx,vy,vz FP32
-# read and x,y,z written (36 B); on the step after a collision the rotated
-# velocities are written and the sorted cell keys read (16 B more), once
-# every csteps = 10 launches
-STREAM_BYTES_PER_PARTICLE = 36.0 + 16.0 / 10.0
+# Algorithmic bytes per particle-step of the FP32 path: streaming reads and
+# writes x and v once per step (2 x 24 B); each collision (every csteps = 10
+# steps) reads x and v once more for the cell sums (24 B / 10).  The kernel
+# roofline charges the streaming launch its 48 B; the step roofline charges
+# whole steps 50.4 B.
+STREAM_BYTES_PER_PARTICLE = 48.0
+STEP_BYTES_PER_PARTICLE = 48.0 + 24.0 / 10.0
 
 
 def cpu_baseline(b, d, st, budget_s=15.0, max_steps=5000):
@@ -199,6 +201,45 @@ def served(n, rate, steps, budget_gb, plain_n=3):
     return out
 
 
+def trace_summary(path, skip, steps):
+    """Timeline of `steps` MD steps from a rocprofv3 kernel trace (CSV),
+    after the first `skip` fused streaming launches: every kernel with its
+    start, duration and the idle gap before it, then the span, the busy
+    time and their per-step averages.  Returns the text."""
+    import csv
+    import re
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+    ks = [(re.sub(r"\(.*", "", r["Kernel_Name"]).replace("void ", ""), int(r["Start_Timestamp"]),
+           int(r["End_Timestamp"])) for r in rows]
+    lead = ("k_step_stream", "k_stream")
+    idx = [i for i, k in enumerate(ks) if k[0].startswith(lead)]
+    if len(idx) < skip + steps:
+        raise SystemExit("trace holds %d streaming launches, fewer than %d" % (len(idx), skip + steps))
+    # from the first kernel after the previous step's last launch (a call's
+    # prologue belongs to the step it precedes) to the last step's end
+    j = idx[skip]
+    while j > 0 and not ks[j - 1][0].startswith(("k_step_near", "k_near")):
+        j -= 1
+    e = idx[skip + steps] if skip + steps < len(idx) else len(ks)
+    while e > 0 and not ks[e - 1][0].startswith(("k_step_near", "k_near", "k_mem")):
+        e -= 1
+    t0, prev, busy, by = ks[j][1], ks[j][1], 0, {}
+    lines = ["%-24s %10s %9s %8s" % ("kernel", "start_us", "dur_us", "gap_us")]
+    for name, a, b in ks[j:e]:
+        lines.append("%-24s %10.1f %9.1f %8.1f" % (name[:24], (a - t0) / 1e3, (b - a) / 1e3, max(0, a - prev) / 1e3))
+        busy += b - a
+        by[name] = by.get(name, 0) + (b - a)
+        prev = max(prev, b)
+    span = prev - t0
+    lines.append("span %.1f us, busy %.1f us, idle %.1f us over %d steps: %.2f us per step "
+                 "(busy %.2f, idle %.2f)" % (span / 1e3, busy / 1e3, (span - busy) / 1e3, steps,
+                                             span / 1e3 / steps, busy / 1e3 / steps, (span - busy) / 1e3 / steps))
+    lines.append("busy time per kernel, us per step:")
+    for name, t in sorted(by.items(), key=lambda kv: -kv[1]):
+        lines.append("  %-24s %8.2f" % (name[:24], t / 1e3 / steps))
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -216,7 +257,13 @@ def main():
     ap.add_argument("--profile-every", type=int, default=4)
     ap.add_argument("--options", type=int, default=-1,
                     help="launch-option index (bit i = irbc_amd.OPTIONS[i]) instead of autotuning")
+    ap.add_argument("--trace-summary", metavar="CSV",
+                    help="summarise a rocprofv3 kernel trace of this benchmark (the --steps steps after "
+                         "--warmup steps) and exit; needs no GPU")
     a = ap.parse_args()
+    if a.trace_summary:
+        print(trace_summary(a.trace_summary, a.warmup, a.steps))
+        return
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the replicas of this workload are independent "
               "and shard by replica, so there is no multi-GPU run to measure (asked for %d)"
@@ -253,6 +300,7 @@ def main():
     value = ns * a.steps / dt
     k_ms = prof["stream_ms"] / max(prof["stream_n"], 1)
     achieved = STREAM_BYTES_PER_PARTICLE * ns / (k_ms * 1e-3) / 1e9
+    step_gbs = STEP_BYTES_PER_PARTICLE * value / 1e9
     opt = dict((name, (tuned >> i) & 1) for i, (name, _) in enumerate(ia.OPTIONS))
     if opt["one_queue"] and opt["membrane_chain"]:
         kname = "k_step_stream<%s>" % ("true" if opt["stream4"] else "false")
@@ -271,15 +319,20 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": load_traffic(tfile), "bytes_per_particle": STREAM_BYTES_PER_PARTICLE,
-                     "kernel": kname, "kernel_ms": k_ms, "kernel_launches_timed": prof["stream_n"]},
+                     "kernel": kname, "kernel_ms": k_ms, "kernel_launches_timed": prof["stream_n"],
+                     "note": "kernel_ms is the whole fused launch: the bulk sweep and, on one queue, "
+                             "the membrane chain's phases riding in it",
+                     "step": {"achieved": step_gbs, "frac": step_gbs / HBM_PEAK_GBS,
+                              "bytes_per_particle_step": STEP_BYTES_PER_PARTICLE,
+                              "ms_per_step": dt * 1e3 / a.steps}},
         "launch_options": dict([(name, (tuned >> i) & 1) for i, (name, _) in enumerate(ia.OPTIONS)],
                                autotune_ms_per_step=[round(x, 4) if x == x else None for x in trials.tolist()]),
         "phases_ms_per_launch": {
             "stream": k_ms,
-            "collision_sequence": prof["collision_ms"] / max(prof["collision_n"], 1),
+            "collision_sequence": prof["collision_ms"] / prof["collision_n"] if prof["collision_n"] else None,
             # on one queue the membrane runs inside the streaming launch
             "membrane": prof["membrane_ms"] / prof["membrane_n"] if prof["membrane_n"] else None,
-            "stream_near": prof["near_ms"] / max(prof["near_n"], 1)},
+            "stream_near": prof["near_ms"] / prof["near_n"] if prof["near_n"] else None},
     }
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(b, d, st)

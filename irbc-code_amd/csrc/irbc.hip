@@ -215,6 +215,11 @@ struct irbc_ctx {
     int near_masks = 1;         /* sub-cell near masks (1) or every particle of a listed cell queued (0) */
     int one_queue = 1;          /* with the chain: the step on one queue, the membrane inside the solvent's launches */
     int ncu = 256;              /* compute units of the device */
+    /* the device state is as the last irbc_md_steps left it (vertex and
+     * triangle kinematics, near masks, drift margins, the next step's grid
+     * shift), so the next call needs no prologue; any call that changes
+     * state clears it */
+    bool primed = false;
     std::map<std::pair<uint64_t, uint64_t>, hipGraphExec_t> graphs;
     int nrep = 0;
     std::vector<HostRep> hr;
@@ -327,6 +332,7 @@ struct irbc_ctx {
     }
     int push(int r)
     {
+        primed = false;
         HIPCHK(hipMemcpyAsync(d_rep + r, &rep[r], sizeof(DevRep), hipMemcpyHostToDevice, st));
         HIPCHK(hipStreamSynchronize(st));
         return IRBC_OK;
@@ -783,6 +789,7 @@ int irbc_set_vertices(irbc_ctx *c, int r, const double *x, const double *y, cons
     double *dd[] = {c->mx, c->my, c->mz, c->mvx, c->mvy, c->mvz, c->mfx, c->mfy, c->mfz};
     for (int k = 0; k < 9; ++k)
         if (ds[k]) HIPCHK(hipMemcpyAsync(dd[k] + R.voff, ds[k], n * 8, hipMemcpyHostToDevice, c->st));
+    c->primed = false;
     const int32_t *is[] = {ix, iy, pid, bondi, bondl, knob};
     int32_t *id[] = {c->mix, c->miy, c->mpid, c->bondi, c->bondl, c->knob};
     for (int k = 0; k < 6; ++k)
@@ -811,6 +818,29 @@ int irbc_get_vertices(irbc_ctx *c, int r, double *x, double *y, double *z, doubl
     return IRBC_OK;
 }
 
+/* particle indices in order of their unshifted cell (a stable counting
+ * sort; positions outside the box are folded in, for the order only) */
+static std::vector<int64_t> cell_order(const DevRep &R, const double *x, const double *y, const double *z, int64_t n)
+{
+    const int Lx = R.Lx, Ly = R.Ly, Lz = R.Lz;
+    const int64_t nc = (int64_t)Lx * Ly * (Lz + 1);
+    std::vector<int32_t> key(n);
+    std::vector<int64_t> start(nc + 1, 0), order(n);
+    auto fold = [](double v, int L) {
+        const double w = v - L * std::floor(v / L);
+        return (w >= 0.0 && w < L) ? std::min(L - 1, (int)w) : 0; /* NaN and inf fold to 0 */
+    };
+    for (int64_t i = 0; i < n; ++i) {
+        const double zz = z[i] + 1.0;
+        const int fz = (zz >= 0.0 && zz < Lz + 1.0) ? std::min(Lz, (int)zz) : 0;
+        key[i] = (int32_t)(((int64_t)fold(x[i], Lx) * Ly + fold(y[i], Ly)) * (Lz + 1) + fz);
+        ++start[key[i] + 1];
+    }
+    for (int64_t k = 0; k < nc; ++k) start[k + 1] += start[k];
+    for (int64_t i = 0; i < n; ++i) order[start[key[i]]++] = i;
+    return order;
+}
+
 int irbc_set_solvent(irbc_ctx *c, int r, int64_t n, const double *x, const double *y,
                      const double *z, const double *vx, const double *vy, const double *vz,
                      const int32_t *indio)
@@ -821,15 +851,27 @@ int irbc_set_solvent(irbc_ctx *c, int r, int64_t n, const double *x, const doubl
     int rc = c->pull(r);
     if (rc) return rc;
     DevRep &R = c->rep[r];
+    /* The particles go to the device in cell order (unshifted cells, key
+     * layout of cell_of): the first streaming steps and the first
+     * collision's sort then touch neighbouring cells together, as every step
+     * does once a collision has sorted them (a random upload order made the
+     * first ten steps twice as slow and the first sort seven times).  No
+     * result depends on the storage order: cell sums are exact integer sums,
+     * every other per-particle step is independent, downloads return
+     * particle order. */
+    const std::vector<int64_t> order = cell_order(R, x, y, z, n);
     std::vector<float> f(n);
     const double *src[] = {x, y, z, vx, vy, vz};
     float *dst[] = {c->xa, c->ya, c->za, c->vxa, c->vya, c->vza};
     for (int k = 0; k < 6; ++k) {
-        for (int64_t i = 0; i < n; ++i) f[i] = (float)src[k][i];
+        for (int64_t i = 0; i < n; ++i) f[i] = (float)src[k][order[i]];
         HIPCHK(hipMemcpy(dst[k] + R.poff, f.data(), n * 4, hipMemcpyHostToDevice));
     }
     std::vector<uint32_t> id(n);
-    for (int64_t i = 0; i < n; ++i) id[i] = (uint32_t)i | ((indio ? (uint32_t)(indio[i] != 0) : 1u) << 31);
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t j = order[i];
+        id[i] = (uint32_t)j | ((indio ? (uint32_t)(indio[j] != 0) : 1u) << 31);
+    }
     HIPCHK(hipMemcpy(c->ida + R.poff, id.data(), n * 4, hipMemcpyHostToDevice));
     std::vector<int32_t> zero(n, 0);
     HIPCHK(hipMemcpy(c->skey2 + R.poff, zero.data(), n * 4, hipMemcpyHostToDevice));
@@ -1334,6 +1376,7 @@ static int ready(irbc_ctx *c, bool need_solvent)
 int irbc_collision_step(irbc_ctx *c)
 {
     if (!c) return fail(IRBC_ERR_ARG, "null context");
+    c->primed = false;
     int rc = ready(c, true);
     if (rc) return rc;
     rc = launch_collision(c, 1, 1);
@@ -1344,6 +1387,7 @@ int irbc_collision_step(irbc_ctx *c)
 int irbc_llist_update(irbc_ctx *c)
 {
     if (!c) return fail(IRBC_ERR_ARG, "null context");
+    c->primed = false;
     int rc = ready(c, false);
     if (rc) return rc;
     rc = launch_llist(c, 1);
@@ -1354,6 +1398,7 @@ int irbc_llist_update(irbc_ctx *c)
 int irbc_stream(irbc_ctx *c)
 {
     if (!c) return fail(IRBC_ERR_ARG, "null context");
+    c->primed = false;
     int rc = ready(c, true);
     if (rc) return rc;
     int rc2 = launch_tri_kin(c, 1);
@@ -1370,6 +1415,7 @@ int irbc_stream(irbc_ctx *c)
 int irbc_classify_solvent(irbc_ctx *c)
 {
     if (!c) return fail(IRBC_ERR_ARG, "null context");
+    c->primed = false;
     int rc = ready(c, true);
     if (rc) return rc;
     k_classify<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->view(c->par), c->mix,
@@ -1381,6 +1427,7 @@ int irbc_classify_solvent(irbc_ctx *c)
 int irbc_membrane_phases(irbc_ctx *c, int phases)
 {
     if (!c) return fail(IRBC_ERR_ARG, "null context");
+    c->primed = false;
     int rc = ready(c, false);
     if (rc) return rc;
     rc = launch_membrane(c, phases, c->st);
@@ -1500,6 +1547,7 @@ static int snap_restore(irbc_ctx *c, const Snapshot &S)
     c->hr = S.hr;
     c->rep = S.rep;
     c->par = S.par;
+    c->primed = false;
     return IRBC_OK;
 }
 
@@ -2128,11 +2176,17 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
     int rc = ready(c, true);
     if (rc) return rc;
     if (nsteps == 0) return IRBC_OK;
-    k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 1);
-    rc = launch_tri_kin(c, 1);
-    if (rc) return rc;
-    rc = launch_cell_mask(c, 1);
-    if (rc) return rc;
+    if (!c->primed) {
+        /* the state came from elsewhere (set calls, stage calls, a
+         * checkpoint): this step's shift, vertex and triangle kinematics and
+         * fresh near masks with zero drift */
+        k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 1);
+        rc = launch_tri_kin(c, 1);
+        if (rc) return rc;
+        rc = launch_cell_mask(c, 1);
+        if (rc) return rc;
+    }
+    c->primed = false; /* until this call's launches are all issued */
     if (c->use_graphs && !c->prof) {
         const int CH = IRBC_GRAPH_STEPS;
         for (int64_t s = 0; s < nsteps;) {
@@ -2147,6 +2201,7 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
             if (rc) return rc;
             s += n;
         }
+        c->primed = true;
         return IRBC_OK;
     }
     for (int64_t s = 0; s < nsteps; ++s) {
@@ -2161,7 +2216,9 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
             if (rc) return rc;
         }
     }
-    return join_chains(c);
+    rc = join_chains(c);
+    if (rc == IRBC_OK) c->primed = true;
+    return rc;
 }
 
 #ifdef IRBC_CHAIN_CLOCK
@@ -2179,6 +2236,7 @@ extern "C" int irbc_debug_chain_clock(irbc_ctx *c, long long *out)
 int irbc_set_option(irbc_ctx *c, int option, int value)
 {
     if (!c) return fail(IRBC_ERR_ARG, "null context");
+    c->primed = false; /* the masks' and queues' use differs between shapes */
     switch (option) {
     case IRBC_OPT_MEMBRANE_FUSED:
         c->mem_fused = value != 0;
