@@ -256,6 +256,24 @@ int irbc_profile(irbc_ctx *ctx, int every);
  * t[6] = total ms of near-membrane stream launches, t[7] = their count */
 int irbc_profile_read(irbc_ctx *ctx, double *t);
 
+/* ---- TotalVA with no allocation (no counterpart in the reference) ---- */
+/* For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * global area and volume of TotalVA() (Src/memfuncs.cpp:161-190) of a
+ * membrane held in the caller's device memory, with the same arithmetic and
+ * summation order as irbc_membrane_va, so the same bits.  x, y, z (FP64) and
+ * ix, iy (periodic images) have nv entries, tris is nt x 3 (0-based), all
+ * on the calling thread's current device; area and volume go to out[0],
+ * out[1] (device memory) in `stream` order (a hipStream_t; NULL: the null
+ * stream).  Calls no allocating routine: the per-block partials live in a
+ * fixed device array of 1024 blocks of 256 triangles, so nt <= 262144
+ * (IRBC_ERR_ARG beyond).  A triangle index outside [0, nv) makes the result
+ * NaN.  Every call on a device shares that device's array: calls that
+ * overlap on one device race. */
+int irbc_membrane_va_static(const double *x, const double *y, const double *z, const int32_t *ix,
+                            const int32_t *iy, int32_t nv, const int32_t *tris, int32_t nt, double Lx, double Ly,
+                            double *out, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

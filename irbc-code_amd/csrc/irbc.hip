@@ -23,6 +23,7 @@
 #include <string>
 #include <map>
 #include <numeric>
+#include <mutex>
 #include <vector>
 
 #include "../../include/irbc.h"
@@ -2307,6 +2308,105 @@ int irbc_profile_read(irbc_ctx *c, double *t)
         t[2 * k] = c->acc_ms[k];
         t[2 * k + 1] = (double)c->acc_n[k];
     }
+    return IRBC_OK;
+}
+
+} /* extern "C" */
+
+/* ---------------- TotalVA with no allocation ---------------- */
+
+/* Per-block area/volume partials of irbc_membrane_va_static: one fixed
+ * array per device, held until the process ends.  Host code never uses
+ * this name as an address (it is a host-side stand-in there): the device
+ * address comes from hipGetSymbolAddress, once per device. */
+#define IRBC_VA_MAX_BLOCKS 1024 /* blocks of IRBC_WG triangles: up to 262144 triangles */
+static __device__ __align__(256) double scratch[2 * IRBC_VA_MAX_BLOCKS];
+
+#define IRBC_VA_MAX_DEVICES 64
+static double *g_scratch_ptr[IRBC_VA_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+/* per triangle block: the unwrapped corners as the membrane step forms
+ * them (x + ix Lx, y + iy Ly, z), Areatri/Voltri and the block's partials,
+ * with the same arithmetic and the same blocks of IRBC_WG triangles as the
+ * context's membrane kernels */
+__global__ __launch_bounds__(IRBC_WG) void k_va_blocks(const double *__restrict__ x, const double *__restrict__ y,
+                                                      const double *__restrict__ z, const int32_t *__restrict__ ix,
+                                                      const int32_t *__restrict__ iy,
+                                                      const int32_t *__restrict__ tris, int nv, int nt, double Lx,
+                                                      double Ly, double *__restrict__ part)
+{
+    __shared__ double red[2][IRBC_WG / 64];
+    const int e = (int)blockIdx.x * IRBC_WG + (int)threadIdx.x;
+    double sa = 0.0, sv = 0.0;
+    if (e < nt) {
+        double P[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int g = tris[3 * e + k];
+            if (g < 0 || g >= nv) { /* a vertex index out of range: the result is NaN */
+                P[k][0] = P[k][1] = P[k][2] = __builtin_nan("");
+                continue;
+            }
+            P[k][0] = x[g] + ix[g] * Lx;
+            P[k][1] = y[g] + iy[g] * Ly;
+            P[k][2] = z[g];
+        }
+        tri_area_volume(P[0], P[1], P[2], sa, sv);
+    }
+    av_block_partials(sa, sv, red, part + 2 * (int64_t)blockIdx.x);
+}
+
+/* the blocks' partials summed in block order, as the membrane step sums them */
+__global__ __launch_bounds__(64) void k_va_sum(const double *__restrict__ part, int nblk, double *__restrict__ out)
+{
+    if (threadIdx.x != 0) return;
+    double a = 0.0, v = 0.0;
+    for (int b = 0; b < nblk; ++b) {
+        a += part[2 * b];
+        v += part[2 * b + 1];
+    }
+    out[0] = a;
+    out[1] = v;
+}
+
+extern "C" {
+
+int irbc_membrane_va_static(const double *x, const double *y, const double *z, const int32_t *ix,
+                            const int32_t *iy, int32_t nv, const int32_t *tris, int32_t nt, double Lx, double Ly,
+                            double *out, void *stream)
+{
+    if (!x || !y || !z || !ix || !iy || !tris || !out || nv < 3 || nt < 1)
+        return fail(IRBC_ERR_ARG, "irbc_membrane_va_static: null array or empty mesh");
+    const int nblk = (nt + IRBC_WG - 1) / IRBC_WG;
+    if (nblk > IRBC_VA_MAX_BLOCKS)
+        return fail(IRBC_ERR_ARG, "irbc_membrane_va_static: %d triangles, at most %d", nt,
+                    IRBC_VA_MAX_BLOCKS * IRBC_WG);
+    if (!(Lx > 0.0) || !(Ly > 0.0)) return fail(IRBC_ERR_ARG, "irbc_membrane_va_static: box lengths must be > 0");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= IRBC_VA_MAX_DEVICES)
+        return fail(IRBC_ERR_ARG, "irbc_membrane_va_static: device %d beyond %d", dev, IRBC_VA_MAX_DEVICES);
+    double *part = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            /* the device's own copy of the array, looked up with that
+             * device current (the calling thread's) */
+            double *p = nullptr;
+            const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return fail(IRBC_ERR_HIP, "irbc_membrane_va_static: hipGetSymbolAddress: %s", hipGetErrorString(e));
+            g_scratch_ptr[dev] = p;
+        }
+        part = g_scratch_ptr[dev];
+    }
+    /* every block writes its partials before the sum reads them: nothing
+     * in the array needs clearing between calls */
+    hipStream_t s = (hipStream_t)stream;
+    k_va_blocks<<<(unsigned)nblk, IRBC_WG, 0, s>>>(x, y, z, ix, iy, tris, nv, nt, Lx, Ly, part);
+    k_va_sum<<<1, 64, 0, s>>>(part, nblk, out);
+    LAUNCHCHK();
     return IRBC_OK;
 }
 

@@ -27,6 +27,7 @@ EXPORTS = (
     "irbc_get_pairs", "irbc_md_steps", "irbc_set_diagnostics", "irbc_profile",
     "irbc_profile_read", "irbc_set_series", "irbc_get_series", "irbc_query", "irbc_set_option",
     "irbc_checkpoint_bytes", "irbc_checkpoint_save", "irbc_checkpoint_load", "irbc_autotune",
+    "irbc_membrane_va_static",
 )
 
 OPT_MEMBRANE_FUSED = 1
@@ -151,6 +152,8 @@ def load():
     L.irbc_checkpoint_save.restype = C.c_int64
     L.irbc_checkpoint_save.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
     L.irbc_checkpoint_load.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
+    L.irbc_membrane_va_static.argtypes = [C.c_void_p] * 5 + [C.c_int32, C.c_void_p, C.c_int32, C.c_double,
+                                                             C.c_double, C.c_void_p, C.c_void_p]
     _lib = L
     return L
 
@@ -175,6 +178,16 @@ def _f64(a):
 
 def _i32(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def membrane_va_static(x, y, z, ix, iy, nv, tris, nt, Lx, Ly, out, stream=0):
+    """TotalVA of a membrane held in caller-owned device memory, with no
+    allocation: every argument but the sizes and box lengths is a device
+    address (an int, e.g. a torch tensor's data_ptr()); area and volume go
+    to out[0], out[1] on the device, ordered on `stream` (0: the null
+    stream)."""
+    _chk(load().irbc_membrane_va_static(x, y, z, ix, iy, int(nv), tris, int(nt), float(Lx), float(Ly), out,
+                                        stream or None))
 
 
 def make_spec(params, n_solvent, nv, ne, nt, seed=1):
