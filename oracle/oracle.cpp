@@ -108,6 +108,49 @@ struct Polar {
     }
 };
 
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as
+ * easy as 1, 2, 3", SC'11): ten rounds of two 32x32->64 multiplies with the
+ * Weyl key schedule.  The counter-stream mode below keys it exactly as the
+ * device's production streams do, so the oracle can run free over many
+ * steps on the same draws: counter {index lo, stream, index hi, sub}, key
+ * {seed lo, seed hi}; a uniform is the top 53 bits of a 64-bit half over
+ * 2^53; a normal pair is Box-Muller of two uniforms. */
+struct Philox {
+    static void block(uint32_t c[4], uint32_t k0, uint32_t k1)
+    {
+        for (int r = 0; r < 10; ++r) {
+            const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+            const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n1 = (uint32_t)p1;
+            const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1, n3 = (uint32_t)p0;
+            c[0] = n0;
+            c[1] = n1;
+            c[2] = n2;
+            c[3] = n3;
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+    }
+    static void u2(uint64_t seed, uint32_t stream, uint64_t index, uint32_t sub, double &u0, double &u1)
+    {
+        uint32_t c[4] = {(uint32_t)index, stream, (uint32_t)(index >> 32), sub};
+        block(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+        const uint64_t a = ((uint64_t)c[0] << 32 | c[1]) >> 11, b = ((uint64_t)c[2] << 32 | c[3]) >> 11;
+        u0 = (double)a * (1.0 / 9007199254740992.0);
+        u1 = (double)b * (1.0 / 9007199254740992.0);
+    }
+    static void n2(uint64_t seed, uint32_t stream, uint64_t index, uint32_t sub, double &z0, double &z1)
+    {
+        double u0, u1;
+        u2(seed, stream, index, sub, u0, u1);
+        const double r = std::sqrt(-2.0 * std::log(1.0 - u0));
+        const double t = 6.283185307179586 * u1;
+        z0 = r * std::cos(t);
+        z1 = r * std::sin(t);
+    }
+};
+/* stream ids of the counter-based draws */
+enum { PX_SHIFT = 1, PX_AXIS = 2, PX_WALL = 3, PX_MEM = 4, PX_BOND = 5 };
+
 inline void cross(const double A[3], const double B[3], double C[3])
 {
     /* cp() (Src/memfuncs.cpp:47-52) */
@@ -173,6 +216,28 @@ struct orc {
     Mt19937 ran;
     Polar sto;
     int tapes = 0;
+    /* counter-based mode: every draw from Philox keyed by (seed, step,
+     * cell or vertex), as the device's production streams */
+    int philox = 0;
+    uint64_t pseed = 0;
+    /* FP32 storage mode: solvent positions and velocities rounded to float
+     * where the device stores them (upload, the rotated velocities before
+     * streaming, the end of each streaming step) */
+    int fp32 = 0;
+    void round_solvent()
+    {
+        if (!fp32) return;
+        const int64_t n = (int64_t)x.size();
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            x[i] = (double)(float)x[i];
+            y[i] = (double)(float)y[i];
+            z[i] = (double)(float)z[i];
+            vx[i] = (double)(float)vx[i];
+            vy[i] = (double)(float)vy[i];
+            vz[i] = (double)(float)vz[i];
+        }
+    }
     double shift_u[3] = {0, 0, 0};
     std::vector<double> cell_u, wall_z, mem_z, bond_u;
     int64_t bond_pos = 0;
@@ -201,6 +266,12 @@ struct orc {
     double d_shift(int k)
     {
         if (tapes) return shift_u[k];
+        if (philox) {
+            double u0, u1, u2, u3;
+            Philox::u2(pseed, PX_SHIFT, (uint64_t)step, 0u, u0, u1);
+            Philox::u2(pseed, PX_SHIFT, (uint64_t)step, 1u, u2, u3);
+            return shift_u[k] = (k == 0 ? u0 : k == 1 ? u1 : u2);
+        }
         double u = ran.uniform();
         shift_u[k] = u;
         return u;
@@ -208,13 +279,25 @@ struct orc {
     double d_cell(int64_t c, int k)
     {
         if (tapes) return cell_u[2 * c + k];
+        if (philox) {
+            double u0, u1;
+            Philox::u2(pseed, PX_AXIS, (uint64_t)step, (uint32_t)c, u0, u1);
+            return cell_u[2 * c + k] = (k ? u1 : u0);
+        }
         double u = ran.uniform();
         cell_u[2 * c + k] = u;
         return u;
     }
+    double px_normal(uint32_t stream, int64_t c, int k) const
+    {
+        double a, b;
+        Philox::n2(pseed, stream, (uint64_t)step, 2u * (uint32_t)c + (uint32_t)(k >> 1), a, b);
+        return (k & 1) ? b : a;
+    }
     double d_wall(int64_t c, int k)
     {
         if (tapes) return wall_z[3 * c + k];
+        if (philox) return wall_z[3 * c + k] = px_normal(PX_WALL, c, k);
         double zz = sto.next();
         wall_z[3 * c + k] = zz;
         return zz;
@@ -222,6 +305,7 @@ struct orc {
     double d_mem(int64_t c, int k)
     {
         if (tapes) return mem_z[3 * c + k];
+        if (philox) return mem_z[3 * c + k] = px_normal(PX_MEM, c, k);
         double zz = sto.next();
         mem_z[3 * c + k] = zz;
         return zz;
@@ -230,6 +314,17 @@ struct orc {
     {
         if (tapes) return bond_pos < (int64_t)bond_u.size() ? bond_u[bond_pos++] : 0.0;
         double u = ran.uniform();
+        bond_u.push_back(u);
+        return u;
+    }
+    /* the k-th draw of vertex v in this step's bond dynamics: keyed by
+     * (v, k) in the counter-based mode, the next draw of the sequence else */
+    double d_bond_at(int v, int k)
+    {
+        if (!philox || tapes) return d_bond();
+        double u0, u1;
+        Philox::u2(pseed, PX_BOND, (uint64_t)step, ((uint32_t)v << 4) | (uint32_t)(k >> 1), u0, u1);
+        const double u = (k & 1) ? u1 : u0;
         bond_u.push_back(u);
         return u;
     }
@@ -375,6 +470,26 @@ void orc_seed_mt(orc *o, int32_t seed_ran, int32_t seed_sto)
 
 void orc_use_tapes(orc *o, int on) { o->tapes = on; }
 
+void orc_use_philox(orc *o, uint64_t seed)
+{
+    o->philox = 1;
+    o->pseed = seed;
+    o->tapes = 0;
+}
+
+void orc_philox_u2(uint64_t seed, uint32_t stream, uint64_t index, uint32_t sub, double *u)
+{
+    Philox::u2(seed, stream, index, sub, u[0], u[1]);
+}
+
+void orc_philox_block(uint32_t *c, uint32_t k0, uint32_t k1) { Philox::block(c, k0, k1); }
+
+void orc_set_fp32(orc *o, int on)
+{
+    o->fp32 = on ? 1 : 0;
+    o->round_solvent();
+}
+
 void orc_set_collision_tape(orc *o, const double *shift_u, const double *cell_u,
                             const double *wall_z, const double *mem_z)
 {
@@ -513,6 +628,7 @@ void orc_set_solvent(orc *o, int64_t n, const double *x, const double *y, const 
         o->indio.assign(indio, indio + n);
     else
         o->indio.assign(n, 1);
+    o->round_solvent();
 }
 
 int64_t orc_n_solvent(const orc *o) { return (int64_t)o->x.size(); }
@@ -867,7 +983,14 @@ void orc_collision(orc *o)
     }
     /* rotations (:700-810) */
 #pragma omp parallel for schedule(static)
-    for (int64_t i = 0; i < ns; ++i) rotate_one(o, o->pid[i], o->vx[i], o->vy[i], o->vz[i]);
+    for (int64_t i = 0; i < ns; ++i) {
+        rotate_one(o, o->pid[i], o->vx[i], o->vy[i], o->vz[i]);
+        if (o->fp32) {
+            o->vx[i] = (double)(float)o->vx[i];
+            o->vy[i] = (double)(float)o->vy[i];
+            o->vz[i] = (double)(float)o->vz[i];
+        }
+    }
     for (int i = 0; i < o->nv; ++i) rotate_one(o, o->mpid[i], o->mvx[i], o->mvy[i], o->mvz[i]);
     rotate_one(o, o->paraid, o->paraV[0], o->paraV[1], o->paraV[2]);
 }
@@ -1194,6 +1317,7 @@ void orc_stream(orc *o, double wallspeed)
         x = wrap_loop(x, Lx);
         y = wrap_loop(y, Ly);
     }
+    o->round_solvent();
 }
 
 /* ---------------- membrane ---------------- */
@@ -1527,7 +1651,7 @@ void orc_bond_dynamics(orc *o)
     o->bond_pos = 0;
     for (int ii = 0; ii < o->nv; ++ii) {
         if (o->bondi[ii] == 1) {
-            double randoff = o->d_bond();
+            double randoff = o->d_bond_at(ii, 0);
             int jj = o->bondl[ii];
             if (o->poff[ii] > randoff) {
                 o->bondi[ii] = 0;
@@ -1538,7 +1662,7 @@ void orc_bond_dynamics(orc *o)
             for (size_t k = 0; k < o->pairs[ii].size(); ++k) {
                 int j = o->pairs[ii][k].second;
                 double pon = o->pon[ii];
-                double randon = o->d_bond();
+                double randon = o->d_bond_at(ii, (int)k);
                 if ((pon > randon) && (o->ligi[j] == 0)) {
                     o->ligi[j] = 1;
                     o->bondi[ii] = 1;

@@ -44,6 +44,15 @@ int64_t orc_ncells(const orc *o);
 /* ---- streams ---- */
 void orc_seed_mt(orc *o, int32_t seed_ran, int32_t seed_sto);
 void orc_use_tapes(orc *o, int on);
+/* counter-based streams: every draw from Philox4x32-10 keyed by (seed, step,
+ * cell or vertex) exactly as the device's production streams are */
+void orc_use_philox(orc *o, uint64_t seed);
+void orc_philox_u2(uint64_t seed, uint32_t stream, uint64_t index, uint32_t sub, double *u);
+void orc_philox_block(uint32_t *ctr4, uint32_t k0, uint32_t k1);
+/* FP32 storage: round the solvent's positions and velocities to float where
+ * the device stores them (now, on upload, after the solvent rotation and
+ * after each streaming step); arithmetic stays FP64 */
+void orc_set_fp32(orc *o, int on);
 void orc_set_collision_tape(orc *o, const double *shift_u, const double *cell_u,
                             const double *wall_z, const double *mem_z);
 void orc_get_collision_tape(const orc *o, double *shift_u, double *cell_u,

@@ -112,6 +112,10 @@ def lib():
         L.orc_init_readme.argtypes = [C.c_void_p, C.c_int32, C.c_int64, C.c_int, _d, C.c_int, _i,
                                       C.c_int, _i, C.c_int, _d]
         L.orc_set_mesh.argtypes = [C.c_void_p, C.c_int, _d, C.c_int, _i, C.c_int, _i]
+        L.orc_use_philox.argtypes = [C.c_void_p, C.c_uint64]
+        L.orc_philox_u2.argtypes = [C.c_uint64, C.c_uint32, C.c_uint64, C.c_uint32, _d]
+        L.orc_philox_block.argtypes = [C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32]
+        L.orc_set_fp32.argtypes = [C.c_void_p, C.c_int]
         _lib = L
     return _lib
 
@@ -122,6 +126,20 @@ def set_threads(n):
 
 def threads():
     return lib().orc_get_threads()
+
+
+def philox_block(ctr, key):
+    """One Philox4x32-10 block: ctr = 4 words, key = 2 words; returns 4 words."""
+    c = (C.c_uint32 * 4)(*[int(v) & 0xFFFFFFFF for v in ctr])
+    lib().orc_philox_block(c, int(key[0]) & 0xFFFFFFFF, int(key[1]) & 0xFFFFFFFF)
+    return [int(v) for v in c]
+
+
+def philox_u2(seed, stream, index, sub):
+    """The two uniforms of the counter-based stream (seed, stream, index, sub)."""
+    u = np.zeros(2)
+    lib().orc_philox_u2(int(seed), int(stream), int(index), int(sub), _ptr(u, _d))
+    return float(u[0]), float(u[1])
 
 
 def read_mesh(vfile, efile, tfile):
@@ -153,6 +171,14 @@ class Oracle:
 
     def use_tapes(self, on=True):
         self.L.orc_use_tapes(self.h, 1 if on else 0)
+
+    def use_philox(self, seed):
+        """Draw every random number from the device's counter-based streams."""
+        self.L.orc_use_philox(self.h, int(seed))
+
+    def set_fp32(self, on=True):
+        """Store the solvent in FP32 where the device does (rounds it now)."""
+        self.L.orc_set_fp32(self.h, 1 if on else 0)
 
     def set_collision_tape(self, shift_u, cell_u, wall_z, mem_z):
         self._tape = [_f64(shift_u), _f64(cell_u), _f64(wall_z), _f64(mem_z)]
