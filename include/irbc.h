@@ -101,9 +101,10 @@ int irbc_get_parasite(irbc_ctx *ctx, int r, double *d, int32_t *i);
 int64_t irbc_n_ligands(const irbc_ctx *ctx, int r);
 int irbc_set_ligands(irbc_ctx *ctx, int r, const int32_t *ligi);
 int irbc_get_ligands(irbc_ctx *ctx, int r, int32_t *ligi);
-/* g[11] = Sarea, Volume, Sarea0, Volume0, theta0, wallspeed, status,
+/* g[12] = Sarea, Volume, Sarea0, Volume0, theta0, wallspeed, status,
  * bond draws of the last step, step, bonded knobs, particles queued for
- * bounce-back in the last step */
+ * bounce-back in the last step, the XCD the replica's last membrane chain
+ * ran on (-1: none, or device-scope hand-offs) */
 int irbc_get_globals(irbc_ctx *ctx, int r, double *g);
 /* the time-loop counters before the top-of-step increment (maincode.cpp:450-475) */
 int irbc_set_counters(irbc_ctx *ctx, int r, int64_t step, int32_t colsteps, int32_t upd);

@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <iterator>
@@ -136,6 +137,7 @@ Sizes plan(const irbc_replica_spec *specs, int n, std::vector<DevRep> *reps,
         R.inc_ent_off = z.inc;
         R.scr_off = z.scr;
         R.rep_index = r;
+        R.chain_xcc_last = -1;
         R.part_off = z.part;
         const int32_t tiles = (int32_t)((s.n_solvent + IRBC_WG - 1) / IRBC_WG);
         const int32_t ctiles = (R.ncell + IRBC_WG - 1) / IRBC_WG;
@@ -394,6 +396,22 @@ static void advance_host(const irbc_params &p, HostRep &h)
     h.step += 1;
 }
 
+/* the same parameters, field by field (the padding after the five
+ * leading ints is not compared) */
+static bool same_params(const irbc_params &a, const irbc_params &b)
+{
+    if (a.Lx != b.Lx || a.Ly != b.Ly || a.Lz != b.Lz || a.csteps != b.csteps || a.uplist != b.uplist) return false;
+    const size_t o = offsetof(irbc_params, navg);
+    return std::memcmp((const char *)&a + o, (const char *)&b + o, sizeof(irbc_params) - o) == 0;
+}
+
+/* time-loop counters as the loop can hold them before its top-of-step
+ * increment */
+static bool counters_valid(const irbc_params &p, int64_t step, int32_t colsteps, int32_t upd)
+{
+    return step >= 0 && colsteps >= -1 && colsteps < p.csteps && upd >= -1 && upd < p.uplist;
+}
+
 static bool host_collides(const irbc_params &p, const HostRep &h) { return h.colsteps + 1 == p.csteps; }
 static bool host_llist(const irbc_params &p, const HostRep &h) { return h.upd + 1 == p.uplist; }
 
@@ -580,6 +598,16 @@ int irbc_sync(irbc_ctx *c)
     int rw = c->wait_all();
     if (rw) return rw;
     HIPCHK(hipMemcpy(c->rep.data(), c->d_rep, c->nrep * sizeof(DevRep), hipMemcpyDeviceToHost));
+    for (int r = 0; r < c->nrep; ++r)
+        if (c->rep[r].status & IRBC_ST_XCD) {
+            /* the dispatch did not keep a replica's chain on one XCD: hand
+             * off at device scope from now on */
+            c->chain_dev = 1;
+            return fail(IRBC_ERR_DEVICE,
+                        "replica %d: membrane-chain workgroups ran on two XCDs (status 0x%x); the launched steps "
+                        "are invalid, the context now hands off at device scope",
+                        r, c->rep[r].status);
+        }
     for (int r = 0; r < c->nrep; ++r)
         if (c->rep[r].status)
             return fail(IRBC_ERR_DEVICE, "replica %d raised device status 0x%x", r, c->rep[r].status);
@@ -980,12 +1008,15 @@ int irbc_get_globals(irbc_ctx *c, int r, double *g)
     g[8] = (double)R.step;
     g[9] = R.nbonds;
     g[10] = R.near_count[c->par ^ 1];
+    g[11] = R.chain_xcc_last;
     return IRBC_OK;
 }
 
 int irbc_set_counters(irbc_ctx *c, int r, int64_t step, int32_t colsteps, int32_t upd)
 {
     CHECK_R(c, r);
+    if (!counters_valid(c->hr[r].spec.params, step, colsteps, upd))
+        return fail(IRBC_ERR_ARG, "irbc_set_counters: need step >= 0, -1 <= colsteps < csteps, -1 <= upd < uplist");
     int rc = c->pull(r);
     if (rc) return rc;
     HostRep &h = c->hr[r];
@@ -1767,6 +1798,62 @@ int64_t irbc_checkpoint_save(irbc_ctx *c, int r, void *buf, int64_t cap)
     return (int64_t)(p - (char *)buf);
 }
 
+/* Everything in a checkpoint that device code uses as an index is checked
+ * on the host before any of it is copied: particle ids (a permutation),
+ * vertex cells, bonds, knobs, ligand occupancy, the parasite's cell and the
+ * cell->triangle lists. */
+static int ckpt_check(const irbc_ctx *c, int r, const CkptHeader &H, const char *p)
+{
+    const DevRep &R = c->rep[r];
+    const HostRep &h = c->hr[r];
+    if (!same_params(H.params, h.spec.params))
+        return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: replica %d has other parameters than the checkpoint", r);
+    if (!counters_valid(h.spec.params, H.step, H.colsteps, H.upd) || (H.tapes != 0 && H.tapes != 1))
+        return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: counters out of range");
+    const int64_t ns = H.ns, nv = H.nv, nlig = H.nlig, ncell = H.ncell;
+    const char *q = p + 6 * ns * 4;
+    std::vector<uint8_t> seen(ns, 0);
+    for (int64_t i = 0; i < ns; ++i) {
+        uint32_t w;
+        std::memcpy(&w, q + 4 * i, 4);
+        const uint32_t id = w & 0x7FFFFFFFu;
+        if ((int64_t)id >= ns || seen[id]) return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: particle ids are not a permutation");
+        seen[id] = 1;
+    }
+    q += ns * 4 + 9 * nv * 8;
+    std::vector<int32_t> vi(6 * nv);
+    std::memcpy(vi.data(), q, (size_t)6 * nv * 4);
+    q += (size_t)6 * nv * 4;
+    for (int64_t v = 0; v < nv; ++v) {
+        const int32_t pid = vi[2 * nv + v], bi = vi[3 * nv + v], bl = vi[4 * nv + v], kb = vi[5 * nv + v];
+        if (pid < 0 || pid >= ncell || (bi != 0 && bi != 1) || (kb != 0 && kb != 1) || (bi == 1 && (bl < 0 || bl >= nlig)))
+            return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: vertex %lld: cell, bond or knob out of range", (long long)v);
+    }
+    for (int64_t l = 0; l < nlig; ++l) {
+        int32_t o;
+        std::memcpy(&o, q + 4 * l, 4);
+        if (o != 0 && o != 1) return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: ligand occupancy out of range");
+    }
+    q += nlig * 4 + 12 * 8;
+    int32_t pi[4];
+    std::memcpy(pi, q, sizeof pi);
+    if (pi[2] < 0 || pi[2] >= ncell) return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: parasite cell out of range");
+    q += sizeof pi;
+    std::vector<int32_t> off(ncell + 1);
+    std::memcpy(off.data(), q, (ncell + 1) * 4);
+    q += (ncell + 1) * 4;
+    if (off[0] != 0 || off[ncell] != H.llen) return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: cell lists do not add up");
+    for (int64_t k = 0; k < ncell; ++k)
+        if (off[k + 1] < off[k]) return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: cell list offsets decrease");
+    for (int64_t e = 0; e < H.llen; ++e) {
+        int32_t t;
+        std::memcpy(&t, q + 4 * e, 4);
+        if (t < 0 || t >= H.nt) return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: cell list entry out of range");
+    }
+    (void)R;
+    return IRBC_OK;
+}
+
 int irbc_checkpoint_load(irbc_ctx *c, int r, const void *buf, int64_t n)
 {
     CHECK_R(c, r);
@@ -1781,8 +1868,11 @@ int irbc_checkpoint_load(irbc_ctx *c, int r, const void *buf, int64_t n)
         H.nt != h.spec.n_triangles || H.ncell != c->rep[r].ncell || H.nlig != c->rep[r].nlig)
         return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: replica %d has other sizes than the checkpoint", r);
     if (n < ckpt_bytes(c, r, H.llen)) return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: truncated buffer");
-    if (H.llen > c->rep[r].llent_cap) return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: cell lists exceed capacity");
-    int rc = c->pull(r);
+    if (H.llen < 0 || H.llen > c->rep[r].llent_cap)
+        return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: cell lists exceed capacity");
+    int rc = ckpt_check(c, r, H, (const char *)buf + sizeof H);
+    if (rc) return rc;
+    rc = c->pull(r);
     if (rc) return rc;
     DevRep &R = c->rep[r];
     const char *p = (const char *)buf + sizeof H;

@@ -8,8 +8,10 @@
  * code advances (so a captured sequence of launches needs no host writes).
  *
  * Precision: solvent positions and velocities are FP32 in HBM and all
- * arithmetic on them is FP64 in registers; per-cell momentum sums are exact
- * 64-bit fixed point; the membrane, parasite and bonds are FP64 throughout.
+ * arithmetic on them is FP64 in registers; per-cell momentum sums are 64-bit
+ * fixed point at 2^-40, order-independent (each term is rounded once, and
+ * is exact for |v| >= 2^-17); the membrane, parasite and bonds are FP64
+ * throughout.
  */
 #ifndef IRBC_DEV_H
 #define IRBC_DEV_H
@@ -31,6 +33,8 @@
 #define IRBC_ST_LLIST_OVERFLOW 4
 #define IRBC_ST_RANGE 8
 #define IRBC_ST_STALL 16       /* a bounded in-kernel wait ran out (the step's results are invalid) */
+#define IRBC_ST_XCD 32         /* a replica's membrane-chain workgroups ran on two XCDs while handing
+                                * off through one XCD's L2 (the step's results are invalid) */
 
 /* counter-based stream ids (Philox4x32-10 counters: {index, stream, hi, lo}) */
 #define IRBC_RS_SHIFT 1u
@@ -119,6 +123,8 @@ struct DevRep {
      * launches: the finishing workgroup clears them) */
     int32_t mem_ticket;
     int32_t mem_done[4];
+    int32_t chain_xcc;      /* XCD id + 1 of the running chain's workgroups (0: none yet) */
+    int32_t chain_xcc_last; /* XCD id of the last finished chain (-1: none) */
     int32_t nlisted;        /* cells that list a triangle (compacted at each list update) */
     int32_t mask_keep;      /* the coming mask launch keeps this replica's masks (decided by the
                              * launch before it, so every mask workgroup sees one decision) */

@@ -310,10 +310,10 @@ class Batch:
         _chk(self.L.irbc_set_ligands(self.h, r, _ptr(a, _i)))
 
     def globals(self, r=0):
-        g = np.zeros(11)
+        g = np.zeros(12)
         _chk(self.L.irbc_get_globals(self.h, r, _ptr(g, _d)))
         return dict(zip(("Sarea", "Volume", "Sarea0", "Volume0", "theta0", "wallspeed", "status",
-                         "bond_draws", "step", "nbonds", "near_queue"), g))
+                         "bond_draws", "step", "nbonds", "near_queue", "chain_xcd"), g))
 
     def set_counters(self, r, step, colsteps, upd):
         _chk(self.L.irbc_set_counters(self.h, r, int(step), int(colsteps), int(upd)))

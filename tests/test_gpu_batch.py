@@ -289,3 +289,24 @@ def test_mask_keep_and_queue_are_deterministic():
     np.testing.assert_array_equal(runs[0][0], runs[1][0])
     for r in range(len(ds)):
         np.testing.assert_array_equal(runs[0][1][r], runs[1][1][r])
+
+
+@pytest.mark.gpu
+def test_membrane_chain_of_a_replica_stays_on_one_xcd():
+    """The chain's XCD-local hand-offs need every workgroup of a replica's
+    chain on one XCD; each workgroup checks its XCD from the hardware
+    register.  On this device the dispatch keeps replica r's chain on XCD
+    r % 8, on one queue and on two, and no replica raises the check."""
+    ds = [en.replica(12, 12, 8, 6, 42, 1.8, 1.0, 20, seed=60 + r) for r in range(10)]
+    for oneq in (1, 0):
+        b = en.batch(ds)
+        try:
+            b.set_option(ia.OPT_ONE_QUEUE, oneq)
+            b.md_steps(12)
+            b.sync()  # raises on IRBC_ST_XCD
+            for r in range(len(ds)):
+                g = b.globals(r)
+                assert g["status"] == 0
+                assert g["chain_xcd"] == r % 8, (oneq, r, g["chain_xcd"])
+        finally:
+            b.close()

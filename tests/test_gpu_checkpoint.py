@@ -45,3 +45,55 @@ def test_checkpoint_refuses_other_sizes():
         b.restore(1, ck)
     with pytest.raises(ia.IrbcError):
         b.restore(0, ck[:100])
+
+
+@pytest.mark.gpu
+def test_checkpoint_refuses_foreign_parameters_and_corrupt_indices():
+    """A checkpoint of a replica with the same sizes but another shear rate
+    is refused, and so is one whose cell->triangle entries or particle ids
+    were corrupted: all are checked on the host before any copy, and the
+    replica then runs on as it was."""
+    d0 = _mix()[0]
+    d1 = dict(d0, shear=d0["shear"] * 0.7)  # same sizes, other uplist and wall speed
+    b = en.batch([d0, d1])
+    try:
+        b.md_steps(5)
+        b.sync()
+        ck = b.checkpoint(0)
+        before = _snapshot(b, 1)
+        with pytest.raises(ia.IrbcError, match="other parameters"):
+            b.restore(1, ck)
+        bad = bytearray(ck)
+        bad[-4:] = (0x7FFFFFFF).to_bytes(4, "little")  # the last cell-list entry
+        with pytest.raises(ia.IrbcError, match="entry out of range"):
+            b.restore(0, bytes(bad))
+        # the particle ids follow the six coordinate arrays
+        spec = b.specs[0]
+        ns, nv, nlig, ncell = spec.n_solvent, spec.n_vertices, b.nlig(0), b.ncells(0)
+        llen = len(b.llist(0)[1])
+        tail = ns * 28 + nv * (9 * 8 + 6 * 4) + nlig * 4 + 12 * 8 + 4 * 4 + (ncell + 1) * 4 + llen * 4
+        ids = len(ck) - tail + 6 * ns * 4
+        bad = bytearray(ck)
+        bad[ids:ids + 4] = bad[ids + 4:ids + 8]  # a duplicated id
+        with pytest.raises(ia.IrbcError, match="permutation"):
+            b.restore(0, bytes(bad))
+        b.restore(0, ck)  # the good one still loads
+        b.md_steps(3)
+        b.sync()
+        after = _snapshot(b, 1)
+        assert after["counters"][0] == before["counters"][0] + 3
+    finally:
+        b.close()
+
+
+@pytest.mark.gpu
+def test_counters_out_of_range_are_refused():
+    b = en.batch(_mix()[:1])
+    try:
+        p = b.specs[0].params
+        for bad in ((0, p.csteps, 0), (0, -2, 0), (0, -1, p.uplist), (-1, -1, 0)):
+            with pytest.raises(ia.IrbcError):
+                b.set_counters(0, *bad)
+        b.set_counters(0, 0, p.csteps - 1, p.uplist - 1)
+    finally:
+        b.close()
