@@ -183,7 +183,7 @@ Sizes plan(const irbc_replica_spec *specs, int n, std::vector<DevRep> *reps,
 int64_t bytes_of(const Sizes &z, int n)
 {
     int64_t b = 0;
-    b += z.slots * (2 * 6 * 4 + 2 * 4 + 4 * 4);           /* particles x2, ids x2, keys, ranks, sorted keys, queue */
+    b += z.slots * (6 * 4 + 4 + 32 + 3 * 4);              /* particles, ids, sort records, keys, ranks, queue */
     b += z.cells * (4 + 8 * 8 + 4 + 4 + 8 * 8 + 4 + 8 + 4); /* count, ctab, vhead, llcount, diag, cursor, near mask, listed */
     b += z.cells * 8 * 8;                                 /* tapes */
     b += z.cells1 * (4 + 4);                              /* start, lloff */
@@ -193,7 +193,7 @@ int64_t bytes_of(const Sizes &z, int n)
     b += (int64_t)z.ne * (5 * 4 + 2 * 8) + (int64_t)z.nt * (3 * 4 + 2 * 8);
     b += z.scr * 32 + (int64_t)z.inc * 4 + (int64_t)(z.nv + n) * 4;
     b += (int64_t)z.nlig * 4 + (int64_t)(z.nligx + z.nligy) * 8;
-    b += (int64_t)z.scanblk * 4 + (int64_t)n * sizeof(DevRep);
+    b += (int64_t)z.scanblk * 8 + 4 + (int64_t)n * sizeof(DevRep);
     return b;
 }
 
@@ -235,11 +235,13 @@ struct irbc_ctx {
     DevRep *d_rep = nullptr;
     int32_t *d_tile_rep = nullptr, *d_ctile_rep = nullptr, *d_rep_first_blk = nullptr;
     ScanBlk *d_blk = nullptr;
-    int32_t *bsum = nullptr;
+    unsigned long long *scanflag = nullptr; /* published sums and prefixes of the look-back scan */
+    int32_t *scanticket = nullptr;
     /* solvent */
-    float *xa, *ya, *za, *vxa, *vya, *vza, *xb, *yb, *zb, *vxb, *vyb, *vzb;
-    uint32_t *ida, *idb;
-    int32_t *skey, *srank, *skey2, *nearq, *deferq;
+    float *xa, *ya, *za, *vxa, *vya, *vza;
+    uint32_t *ida;
+    float *rec; /* the collision's sorted records, 8 floats per slot */
+    int32_t *skey, *srank, *nearq, *deferq;
     /* cells */
     int32_t *ccount, *cstart, *vhead, *lloff, *llcount, *llent;
     double *ctab;
@@ -479,10 +481,11 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
 #define A(p, n_)                                \
     if (rc == IRBC_OK) rc = c->alloc(p, (n_));
     A(c->d_rep, n) A(c->d_tile_rep, z.tiles) A(c->d_ctile_rep, z.ctiles)
-    A(c->d_rep_first_blk, n + 1) A(c->d_blk, z.scanblk) A(c->bsum, z.scanblk)
+    A(c->d_rep_first_blk, n + 1) A(c->d_blk, z.scanblk) A(c->scanflag, z.scanblk)
+    A(c->scanticket, 1)
     A(c->xa, z.slots) A(c->ya, z.slots) A(c->za, z.slots) A(c->vxa, z.slots) A(c->vya, z.slots) A(c->vza, z.slots)
-    A(c->xb, z.slots) A(c->yb, z.slots) A(c->zb, z.slots) A(c->vxb, z.slots) A(c->vyb, z.slots) A(c->vzb, z.slots)
-    A(c->ida, z.slots) A(c->idb, z.slots) A(c->skey, z.slots) A(c->srank, z.slots) A(c->skey2, z.slots) A(c->nearq, z.slots)
+    A(c->rec, 8 * z.slots)
+    A(c->ida, z.slots) A(c->skey, z.slots) A(c->srank, z.slots) A(c->nearq, z.slots)
     A(c->deferq, 2 * z.slots)
     A(c->ccount, z.cells) A(c->cstart, z.cells1) A(c->vhead, z.cells) A(c->lloff, z.cells1)
     A(c->llcount, z.cells) A(c->llent, z.llent) A(c->ctab, z.cells * 8) A(c->cmask, z.cells) A(c->listed, z.cells)
@@ -537,10 +540,9 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
         R.offrate = p.offrate; R.Pi = p.Pi;
         R.pon = 1.0 - std::exp(-p.onrate * p.dtmd);
         R.seed = s.seed;
-        R.cur = 0;
         R.xa = c->xa; R.ya = c->ya; R.za = c->za; R.vxa = c->vxa; R.vya = c->vya; R.vza = c->vza;
-        R.xb = c->xb; R.yb = c->yb; R.zb = c->zb; R.vxb = c->vxb; R.vyb = c->vyb; R.vzb = c->vzb;
-        R.ida = c->ida; R.idb = c->idb;
+        R.ida = c->ida;
+        R.rec = c->rec;
         top_of_step(p, 0, -1, 0, R);
         /* ligand grid coordinates (maincode.cpp:77-93) */
         for (double m = 0.0; m < p.Lx; m += p.lig_spacing) lx.push_back(m);
@@ -902,9 +904,8 @@ int irbc_set_solvent(irbc_ctx *c, int r, int64_t n, const double *x, const doubl
         id[i] = (uint32_t)j | ((indio ? (uint32_t)(indio[j] != 0) : 1u) << 31);
     }
     HIPCHK(hipMemcpy(c->ida + R.poff, id.data(), n * 4, hipMemcpyHostToDevice));
-    std::vector<int32_t> zero(n, 0);
-    HIPCHK(hipMemcpy(c->skey2 + R.poff, zero.data(), n * 4, hipMemcpyHostToDevice));
-    R.cur = 0;
+    /* no collision yet: the records' keys (the particles' last cell keys) read 0 */
+    HIPCHK(hipMemset(c->rec + 8 * R.poff, 0, (size_t)n * 32));
     R.rotate_pending = 0;
     c->hr[r].solvent = true;
     return c->push(r);
@@ -916,7 +917,7 @@ int irbc_get_solvent(irbc_ctx *c, int r, double *x, double *y, double *z, double
     CHECK_R(c, r);
     const int64_t n = c->hr[r].spec.n_solvent;
     HIPCHK(c->sync_both());
-    k_unpermute<<<(unsigned)((n + IRBC_WG - 1) / IRBC_WG), IRBC_WG, 0, c->st>>>(c->d_rep, r, c->skey2,
+    k_unpermute<<<(unsigned)((n + IRBC_WG - 1) / IRBC_WG), IRBC_WG, 0, c->st>>>(c->d_rep, r,
                                                                                c->stage_f, c->stage_i);
     LAUNCHCHK();
     HIPCHK(c->sync_both());
@@ -1140,9 +1141,8 @@ int irbc_set_bond_tape(irbc_ctx *c, int r, const double *u, int64_t n)
 static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t *out)
 {
     const unsigned nb = (unsigned)c->z.scanblk;
-    k_scan_reduce<<<nb, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_blk, staged, which, in, c->bsum);
-    k_scan_top<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_blk, c->d_rep_first_blk, staged, which, c->bsum);
-    k_scan_apply<<<nb, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_blk, staged, which, in, c->bsum, out);
+    k_scan_lb<<<nb, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_blk, (int)nb, staged, which, in, out, c->scanflag,
+                                         c->scanticket);
     LAUNCHCHK();
     return IRBC_OK;
 }
@@ -1164,20 +1164,23 @@ static int launch_collision(irbc_ctx *c, int staged, int diag)
     const int xc = c->xcd_chunk;
     MemDev M = c->mem(c->par);
     const unsigned nvb = IRBC_NXCD * ((unsigned)(c->nrep + IRBC_NXCD - 1) / IRBC_NXCD);
-    k_bin<<<nvb + (xc ? xcd_grid(T) : T), IRBC_WG, 0, c->st>>>(c->d_rep, c->nrep, (int)nvb, c->d_tile_rep, (int)T, xc,
-                                                              staged, c->skey, c->srank, c->ccount, M, c->vhead,
-                                                              c->vnext);
+    /* the cell-draw blocks keep the particle tiles' XCD chunking intact:
+     * their count is a multiple of IRBC_NXCD */
+    const unsigned ndb = xcd_grid(CT);
+    k_bin<<<nvb + ndb + (xc ? xcd_grid(T) : T), IRBC_WG, 0, c->st>>>(
+        c->d_rep, c->nrep, (int)nvb, c->d_tile_rep, (int)T, xc, staged, c->skey, c->srank, c->ccount, M, c->vhead,
+        c->vnext, c->d_ctile_rep, (int)CT, (int)ndb, c->cell_u, c->ctab, c->wall_z, diag, c->diag);
     int rc = launch_scan(c, staged, 0, c->ccount, c->cstart);
     if (rc) return rc;
     k_scatter<<<xc ? xcd_grid(T) : T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, xc, staged, c->skey,
-                                                           c->srank, c->cstart, c->skey2);
-    k_cell<<<xc ? xcd_grid(IRBC_CELL_LANES * CT) : IRBC_CELL_LANES * CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, xc, staged, c->cstart,
+                                                           c->srank, c->cstart);
+    k_cell<<<xc ? xcd_grid(CT) : CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, xc, staged, c->cstart,
                                                          c->vhead,
                                                 c->vnext,
-                                      c->mvx, c->mvy, c->mvz, c->cell_u, c->wall_z, c->mem_z, c->ctab,
+                                      c->mvx, c->mvy, c->mvz, c->wall_z, c->mem_z, c->ctab,
                                       diag, c->diag, c->par);
     k_vrot<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, staged, M, c->ctab);
-    if (staged) k_rotate<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->skey2, c->ctab);
+    if (staged) k_rotate<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->ctab);
     LAUNCHCHK();
     return IRBC_OK;
 }
@@ -1279,7 +1282,7 @@ static int launch_bulk(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1, bool tri_kin)
     const TriKin tk{M.t1, M.t2, M.t3, M.vkin, M.tri_kin, M.tri_sph};
     const int tkb = std::max(1, (c->z.maxnt + IRBC_WG - 1) / IRBC_WG);
     const int ntk = tri_kin ? tkb * c->nrep : 0;
-    const int32_t *tr = c->d_tile_rep, *sk = c->skey2, *lo = c->lloff;
+    const int32_t *tr = c->d_tile_rep, *lo = c->lloff;
     const double *ct = c->ctab;
     const uint64_t *cm = c->cmask;
     if (c->stream4) {
@@ -1287,19 +1290,19 @@ static int launch_bulk(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1, bool tri_kin)
         if (c->stream_grid) G = std::min(G, cap);
         G += (unsigned)ntk;
         if (e0)
-            hipExtLaunchKernelGGL(k_stream4, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep, tr, (int)T, sk, ct,
+            hipExtLaunchKernelGGL(k_stream4, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep, tr, (int)T, ct,
                                   lo, cm, c->nearq, c->par, tk, tkb, ntk);
         else
-            k_stream4<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, tr, (int)T, sk, ct, lo, cm, c->nearq, c->par, tk, tkb, ntk);
+            k_stream4<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, tr, (int)T, ct, lo, cm, c->nearq, c->par, tk, tkb, ntk);
     } else {
         const bool capped = c->stream_grid && T > cap;
         const unsigned G = (capped ? cap : T) + (unsigned)ntk;
         auto *kern = capped ? k_stream<true> : k_stream<false>;
         if (e0)
-            hipExtLaunchKernelGGL(kern, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep, tr, (int)T, sk, ct, lo,
+            hipExtLaunchKernelGGL(kern, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep, tr, (int)T, ct, lo,
                                   cm, c->nearq, c->par, tk, tkb, ntk);
         else
-            kern<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, tr, (int)T, sk, ct, lo, cm, c->nearq, c->par, tk, tkb, ntk);
+            kern<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, tr, (int)T, ct, lo, cm, c->nearq, c->par, tk, tkb, ntk);
     }
     LAUNCHCHK();
     return IRBC_OK;
@@ -1751,14 +1754,12 @@ int64_t irbc_checkpoint_save(irbc_ctx *c, int r, void *buf, int64_t cap)
     char *p = (char *)buf;
     std::memcpy(p, &H, sizeof H);
     p += sizeof H;
-    const bool cur = R.cur != 0;
-    const float *sf[] = {cur ? c->xb : c->xa, cur ? c->yb : c->ya, cur ? c->zb : c->za,
-                         cur ? c->vxb : c->vxa, cur ? c->vyb : c->vya, cur ? c->vzb : c->vza};
+    const float *sf[] = {c->xa, c->ya, c->za, c->vxa, c->vya, c->vza};
     for (int k = 0; k < 6; ++k) {
         HIPCHK(hipMemcpy(p, sf[k] + R.poff, H.ns * 4, hipMemcpyDeviceToHost));
         p += H.ns * 4;
     }
-    HIPCHK(hipMemcpy(p, (cur ? c->idb : c->ida) + R.poff, H.ns * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(p, c->ida + R.poff, H.ns * 4, hipMemcpyDeviceToHost));
     p += H.ns * 4;
     const double *vd[] = {c->mx, c->my, c->mz, c->mvx, c->mvy, c->mvz, c->mfx, c->mfy, c->mfz};
     for (int k = 0; k < 9; ++k) {
@@ -1916,9 +1917,7 @@ int irbc_checkpoint_load(irbc_ctx *c, int r, const void *buf, int64_t n)
     HIPCHK(hipMemcpy(c->lloff + R.coff1, off.data(), (H.ncell + 1) * 4, hipMemcpyHostToDevice));
     if (H.llen > 0) HIPCHK(hipMemcpy(c->llent + R.llent_off, p, H.llen * 4, hipMemcpyHostToDevice));
     /* the particle order of the checkpoint, no pending rotation */
-    std::vector<int32_t> zero(H.ns, 0);
-    HIPCHK(hipMemcpy(c->skey2 + R.poff, zero.data(), H.ns * 4, hipMemcpyHostToDevice));
-    R.cur = 0;
+    HIPCHK(hipMemset(c->rec + 8 * R.poff, 0, (size_t)H.ns * 32));
     R.rotate_pending = 0;
     R.seed = H.seed;
     R.tapes = H.tapes;
@@ -2108,7 +2107,7 @@ static int launch_step_stream(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1, bool tr
     const int ntk = tri_kin ? tkb * c->nrep : 0;
     const int ntkp = IRBC_NXCD * ((ntk + IRBC_NXCD - 1) / IRBC_NXCD); /* the bulk starts on XCD 0 */
     const int T = (int)c->z.tiles;
-    const int32_t *tr = c->d_tile_rep, *sk = c->skey2, *lo = c->lloff;
+    const int32_t *tr = c->d_tile_rep, *lo = c->lloff;
     const double *ct = c->ctab;
     const uint64_t *cm = c->cmask;
     const bool four = c->stream4 != 0;
@@ -2120,9 +2119,9 @@ static int launch_step_stream(irbc_ctx *c, hipEvent_t e0, hipEvent_t e1, bool tr
     auto *kern = four ? k_step_stream<true> : k_step_stream<false>;
     if (e0)
         hipExtLaunchKernelGGL(kern, dim3(G), dim3(IRBC_WG), 0, c->st, e0, e1, 0, c->d_rep, c->nrep, phases, Mm, NB,
-                              CB, tr, T, sk, ct, lo, cm, c->nearq, c->par, tk, tkb, ntk, ntkp, xs);
+                              CB, tr, T, ct, lo, cm, c->nearq, c->par, tk, tkb, ntk, ntkp, xs);
     else
-        kern<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, c->nrep, phases, Mm, NB, CB, tr, T, sk, ct, lo, cm, c->nearq, c->par,
+        kern<<<G, IRBC_WG, 0, c->st>>>(c->d_rep, c->nrep, phases, Mm, NB, CB, tr, T, ct, lo, cm, c->nearq, c->par,
                                        tk, tkb, ntk, ntkp, xs);
     LAUNCHCHK();
     return IRBC_OK;

@@ -94,8 +94,9 @@ struct DevRep {
     int64_t step;           /* index of the step about to run */
     int32_t colsteps, upd;  /* counters after this step's increment */
     int32_t collide, llist; /* this step runs a collision / a list update */
-    int32_t rotate_pending; /* collision done, solvent rotation fused into the next stream */
-    int32_t cur;            /* which particle buffer holds the state (0 = a, 1 = b) */
+    int32_t rotate_pending; /* collision done: the sorted state is in the records (rec), rotated and
+                             * written back by the next streaming pass */
+    int32_t rec_pad;
     double wall2[2];        /* top-wall speed of this and the next step, by step parity (the
                              * membrane part of step s writes that of step s+1) */
     double shift[3];
@@ -123,6 +124,8 @@ struct DevRep {
      * launches: the finishing workgroup clears them) */
     int32_t mem_ticket;
     int32_t mem_done[4];
+    uint32_t scan_epoch;    /* advanced before each of the replica's cell scans (see k_scan_lb) */
+    int32_t scan_pad;
     int32_t chain_xcc;      /* XCD id + 1 of the running chain's workgroups (0: none yet) */
     int32_t chain_xcc_last; /* XCD id of the last finished chain (-1: none) */
     int32_t nlisted;        /* cells that list a triangle (compacted at each list update) */
@@ -134,8 +137,14 @@ struct DevRep {
     double *series;
     /* ---- particle buffers (packed arrays, this replica at poff) ---- */
     float *xa, *ya, *za, *vxa, *vya, *vza;
-    float *xb, *yb, *zb, *vxb, *vyb, *vzb;
-    uint32_t *ida, *idb;    /* particle id | inside-flag << 31 */
+    uint32_t *ida;          /* particle id | inside-flag << 31 */
+    /* The last collision's counting sort: one 32-byte record per slot, in
+     * cell order (vx vy vz, the shifted-cell key as bits, x y z, the id as
+     * bits: k_cell reads only the first 16 bytes).
+     * Between a collision and the streaming pass after it the state lives
+     * here; after that pass the SoA arrays hold it in the same order, and
+     * the records' keys stay the particles' last cell keys. */
+    float *rec;
 };
 
 #ifdef __HIPCC__
