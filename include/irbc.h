@@ -205,6 +205,18 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
  *   membrane finish in the next; 0: the membrane on a second HIP stream with
  *   events between the two.  Same results. */
 #define IRBC_OPT_ONE_QUEUE 11
+/* IRBC_OPT_CLASSIFY_ALL  1: irbc_classify_solvent tests every triangle for
+ *   every particle, as sortsolvent_initial1 does; 0 (default): only the
+ *   triangles whose (y, z) box holds the particle's (y, z), the only ones
+ *   its +x ray can count.  Same flags. */
+#define IRBC_OPT_CLASSIFY_ALL 12
+/* IRBC_OPT_ALIGN_LISTS  1: a replica updates its cell->triangle lists at
+ *   the last collision step before its uplist deadline (or at the
+ *   deadline), so that list work falls on collision steps; 0 (default): the
+ *   reference's cadence, every uplist steps.  The same hits while the
+ *   membrane moves less than the lists' margin between updates, the
+ *   reference's own assumption. */
+#define IRBC_OPT_ALIGN_LISTS 13
 int irbc_set_option(irbc_ctx *ctx, int option, int value);
 /* Times `steps` MD steps under 48 combinations of the options above on
  * this context's own replicas, then keeps the fastest: the membrane as the

@@ -217,6 +217,8 @@ struct irbc_ctx {
     int chain_dev = 0;          /* its hand-offs at device scope (1) or XCD-local (0) */
     int near_masks = 1;         /* sub-cell near masks (1) or every particle of a listed cell queued (0) */
     int one_queue = 1;          /* with the chain: the step on one queue, the membrane inside the solvent's launches */
+    int classify_all = 0;       /* classification tests every triangle (1) or the binned ones (0) */
+    int align_lists = 0;        /* list updates at collision steps (lists_due) */
     int ncu = 256;              /* compute units of the device */
     /* the device state is as the last irbc_md_steps left it (vertex and
      * triangle kinematics, near masks, drift margins, the next step's grid
@@ -384,15 +386,16 @@ static void top_of_step(const irbc_params &p, int64_t step, int32_t colsteps_pre
     R.upd = upd_pre + 1;
     R.collide = R.colsteps == p.csteps;
     if (R.collide) R.colsteps = 0;
-    R.llist = R.upd == p.uplist;
+    R.llist = lists_due(R.upd, R.collide != 0, p.csteps, p.uplist, R.align_lists);
     if (R.llist) R.upd = 0;
 }
 
-static void advance_host(const irbc_params &p, HostRep &h)
+static void advance_host(const irbc_params &p, HostRep &h, int align)
 {
     int32_t c = h.colsteps + 1, u = h.upd + 1;
-    if (c == p.csteps) c = 0;
-    if (u == p.uplist) u = 0;
+    const bool col = c == p.csteps;
+    if (col) c = 0;
+    if (lists_due(u, col, p.csteps, p.uplist, align)) u = 0;
     h.colsteps = c;
     h.upd = u;
     h.step += 1;
@@ -415,7 +418,10 @@ static bool counters_valid(const irbc_params &p, int64_t step, int32_t colsteps,
 }
 
 static bool host_collides(const irbc_params &p, const HostRep &h) { return h.colsteps + 1 == p.csteps; }
-static bool host_llist(const irbc_params &p, const HostRep &h) { return h.upd + 1 == p.uplist; }
+static bool host_llist(const irbc_params &p, const HostRep &h, int align)
+{
+    return lists_due(h.upd + 1, h.colsteps + 1 == p.csteps, p.csteps, p.uplist, align);
+}
 
 extern "C" {
 
@@ -1453,9 +1459,67 @@ int irbc_classify_solvent(irbc_ctx *c)
     c->primed = false;
     int rc = ready(c, true);
     if (rc) return rc;
-    k_classify<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->view(c->par), c->mix,
-                                                           c->miy);
-    LAUNCHCHK();
+    if (c->classify_all) {
+        k_classify<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->view(c->par), c->mix,
+                                                               c->miy);
+        LAUNCHCHK();
+        return irbc_sync(c);
+    }
+    /* triangles binned by their (y, z) box: a set-up call, so its
+     * temporaries are allocated here and freed at the end */
+    const int n = c->nrep;
+    std::vector<int64_t> soff(n), eoff(n);
+    int64_t nbt = 0;
+    for (int r = 0; r < n; ++r) {
+        soff[r] = nbt;
+        nbt += (int64_t)c->rep[r].Ly * (c->rep[r].Lz + 2);
+    }
+    int64_t *d_soff = nullptr, *d_eoff = nullptr;
+    int32_t *d_cnt = nullptr, *d_start = nullptr, *d_ent = nullptr;
+    auto release = [&]() {
+        hipFree(d_soff);
+        hipFree(d_eoff);
+        hipFree(d_cnt);
+        hipFree(d_start);
+        hipFree(d_ent);
+    };
+    if (hipMalloc(&d_soff, n * 8) != hipSuccess || hipMalloc(&d_eoff, n * 8) != hipSuccess ||
+        hipMalloc(&d_cnt, nbt * 4) != hipSuccess || hipMalloc(&d_start, (nbt + n) * 4) != hipSuccess) {
+        release();
+        return fail(IRBC_ERR_OOM, "irbc_classify_solvent: no room for the triangle bins");
+    }
+    rc = IRBC_OK;
+    std::vector<int32_t> start(nbt + n);
+    const dim3 g((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), (unsigned)n);
+    const MemView V = c->view(c->par);
+    if (hipMemcpy(d_soff, soff.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemsetAsync(d_cnt, 0, nbt * 4, c->st) != hipSuccess)
+        rc = fail(IRBC_ERR_HIP, "irbc_classify_solvent: set-up copies failed");
+    if (!rc) {
+        k_cls_bins<<<g, IRBC_WG, 0, c->st>>>(c->d_rep, V, 0, d_soff, d_cnt, nullptr, nullptr, nullptr);
+        k_cls_scan<<<(unsigned)n, IRBC_WG, 0, c->st>>>(c->d_rep, d_soff, d_cnt, d_start);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->st) != hipSuccess ||
+            hipMemcpy(start.data(), d_start, (nbt + n) * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(IRBC_ERR_HIP, "irbc_classify_solvent: counting the bins failed");
+    }
+    int64_t tot = 0;
+    for (int r = 0; r < n && !rc; ++r) {
+        eoff[r] = tot;
+        tot += start[soff[r] + r + (int64_t)c->rep[r].Ly * (c->rep[r].Lz + 2)];
+    }
+    if (!rc && (hipMalloc(&d_ent, std::max<int64_t>(tot, 1) * 4) != hipSuccess))
+        rc = fail(IRBC_ERR_OOM, "irbc_classify_solvent: no room for the bin entries");
+    if (!rc && hipMemcpy(d_eoff, eoff.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(IRBC_ERR_HIP, "irbc_classify_solvent: set-up copies failed");
+    if (!rc) {
+        k_cls_bins<<<g, IRBC_WG, 0, c->st>>>(c->d_rep, V, 1, d_soff, d_cnt, d_start, d_eoff, d_ent);
+        k_classify_binned<<<(unsigned)c->z.tiles, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, V, c->mix, c->miy,
+                                                                      d_soff, d_start, d_eoff, d_ent);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->st) != hipSuccess)
+            rc = fail(IRBC_ERR_HIP, "irbc_classify_solvent: binned classification failed");
+    }
+    release();
+    if (rc) return rc;
     return irbc_sync(c);
 }
 
@@ -1468,7 +1532,7 @@ int irbc_membrane_phases(irbc_ctx *c, int phases)
     rc = launch_membrane(c, phases, c->st);
     if (rc) return rc;
     if (phases & PH_COUNTERS) {
-        for (int r = 0; r < c->nrep; ++r) advance_host(c->hr[r].spec.params, c->hr[r]);
+        for (int r = 0; r < c->nrep; ++r) advance_host(c->hr[r].spec.params, c->hr[r], c->align_lists);
         c->par ^= 1;
     }
     return irbc_sync(c);
@@ -2207,8 +2271,8 @@ static void next_flags(irbc_ctx *c, std::vector<HostRep> &hr, bool &col, bool &l
     col = ll = false;
     for (int r = 0; r < c->nrep; ++r) {
         col |= host_collides(hr[r].spec.params, hr[r]);
-        ll |= host_llist(hr[r].spec.params, hr[r]);
-        advance_host(hr[r].spec.params, hr[r]);
+        ll |= host_llist(hr[r].spec.params, hr[r], c->align_lists);
+        advance_host(hr[r].spec.params, hr[r], c->align_lists);
     }
 }
 
@@ -2229,7 +2293,7 @@ static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         const int par0 = c->par;
-        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+        HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
         int rc = IRBC_OK;
         for (int k = 0; k < n && rc == IRBC_OK; ++k) {
             const bool col = (flags >> (2 * k)) & 1, ll = (flags >> (2 * k + 1)) & 1;
@@ -2249,7 +2313,12 @@ static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
         const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         if (ei != hipSuccess) return fail(IRBC_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ei));
-        if (c->graphs.size() >= 64) { /* bounded cache: drop everything */
+        if (c->graphs.size() >= 64) {
+            /* bounded cache: drop everything, once nothing launched from it
+             * is still queued or running (destroying an executable frees
+             * its kernel-argument memory, which a launch in flight still
+             * reads) */
+            HIPCHK(c->sync_both());
             for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
             c->graphs.clear();
         }
@@ -2361,6 +2430,24 @@ int irbc_set_option(irbc_ctx *c, int option, int value)
     case IRBC_OPT_ONE_QUEUE:
         c->one_queue = value != 0;
         return IRBC_OK;
+    case IRBC_OPT_CLASSIFY_ALL:
+        c->classify_all = value != 0;
+        return IRBC_OK;
+    case IRBC_OPT_ALIGN_LISTS: {
+        /* the device's copy of the rule for every replica, and the step
+         * about to run re-decided under it, as the host mirror decides */
+        HIPCHK(c->sync_both());
+        HIPCHK(hipMemcpy(c->rep.data(), c->d_rep, c->nrep * sizeof(DevRep), hipMemcpyDeviceToHost));
+        c->align_lists = value != 0;
+        for (int r = 0; r < c->nrep; ++r) {
+            DevRep &R = c->rep[r];
+            const HostRep &h = c->hr[r];
+            R.align_lists = c->align_lists;
+            top_of_step(h.spec.params, h.step, h.colsteps, h.upd, R);
+        }
+        HIPCHK(hipMemcpy(c->d_rep, c->rep.data(), c->nrep * sizeof(DevRep), hipMemcpyHostToDevice));
+        return IRBC_OK;
+    }
     default:
         return fail(IRBC_ERR_ARG, "irbc_set_option: unknown option %d", option);
     }

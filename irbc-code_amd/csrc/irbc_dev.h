@@ -125,7 +125,7 @@ struct DevRep {
     int32_t mem_ticket;
     int32_t mem_done[4];
     uint32_t scan_epoch;    /* advanced before each of the replica's cell scans (see k_scan_lb) */
-    int32_t scan_pad;
+    int32_t align_lists;    /* cell->triangle lists at collision steps (lists_due) */
     int32_t chain_xcc;      /* XCD id + 1 of the running chain's workgroups (0: none yet) */
     int32_t chain_xcc_last; /* XCD id of the last finished chain (-1: none) */
     int32_t nlisted;        /* cells that list a triangle (compacted at each list update) */
@@ -149,6 +149,21 @@ struct DevRep {
 
 #ifdef __HIPCC__
 #include <hip/hip_runtime.h>
+
+/* Does this step (upd_after steps since the last list update, after the
+ * top-of-step increment) update the cell->triangle lists?  The reference's
+ * cadence: every uplist steps (maincode.cpp:460, 501-505).  Aligned: also
+ * at the last collision step before that deadline, so that list updates
+ * (and the near masks they rebuild) fall on collision steps; a batch whose
+ * replicas have different uplist then runs its list work only beside its
+ * collisions.  A list built sooner holds every triangle the reference's
+ * older list holds that a particle can still reach within the deadline, so
+ * the hits are the same while the membrane moves less than the lists'
+ * margin between updates (the reference's own assumption). */
+__host__ __device__ inline bool lists_due(int32_t upd_after, bool collide, int32_t csteps, int32_t uplist, int align)
+{
+    return upd_after == uplist || (align && collide && upd_after + csteps > uplist);
+}
 
 /* ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) ---- */
 struct u4 {

@@ -41,6 +41,8 @@ OPT_MEMBRANE_CHAIN = 8
 OPT_CHAIN_DEVICE_SCOPE = 9
 OPT_NEAR_MASKS = 10
 OPT_ONE_QUEUE = 11
+OPT_CLASSIFY_ALL = 12
+OPT_ALIGN_LISTS = 13
 OPTIONS = (("membrane_fused", 1), ("near_merged", 3), ("graphs", 2), ("xcd_chunk", 4), ("stream4", 5),
            ("stream_grid", 6), ("mem_after_stream", 7), ("membrane_chain", 8), ("one_queue", 11))
 

@@ -29,6 +29,7 @@ A replica's result does not depend on what it was batched with or when:
 every per-replica computation is independent of the rest of its batch and
 of how its steps are split into calls.
 """
+import concurrent.futures as cf
 import time
 from collections import deque
 
@@ -61,9 +62,13 @@ class Request:
 class _Group:
     """One context holding the requests admitted together."""
 
-    def __init__(self, reqs, now):
+    def __init__(self, reqs, now, graphs=False):
         self.reqs = reqs
         self.batch = ia.Batch([r.spec for r in reqs])
+        # a mixed micro-batch's list updates fall on many different steps, so
+        # its chunks take many shapes: captured graphs would rarely be
+        # replayed
+        self.batch.set_option(ia.OPT_GRAPHS, 1 if graphs else 0)
         for i, r in enumerate(reqs):
             en.load(self.batch, i, r.replica)
             if r.series_every > 0:
@@ -81,21 +86,24 @@ class _Group:
 
 
 class Server:
-    def __init__(self, budget_bytes, chunk=250, max_group=64, max_active=2, window=0.0):
+    def __init__(self, budget_bytes, chunk=250, max_group=64, max_active=2, window=0.0, builders=1, graphs=False):
         self.budget = int(budget_bytes)
         self.chunk = int(chunk)
         self.max_group = int(max_group)
         self.max_active = int(max_active)
         self.window = float(window)
+        self.builders = int(builders)
+        self.graphs = bool(graphs)
         self.used = 0
         self.active = []
+        self.building = []  # (future of a _Group, its reserved bytes)
         self.stats = {"groups": 0, "max_active_groups": 0, "max_resident_replicas": 0}
 
     def _admit(self, queue, now):
         """Arrived requests, in arrival order, while they fit the budget.
         Requests of one group share a step count (the group advances
         together); a request that does not fit waits for memory."""
-        while queue and queue[0].arrival <= now and len(self.active) < self.max_active:
+        while queue and queue[0].arrival <= now and len(self.active) + len(self.building) < self.max_active:
             head = queue[0]
             if now - head.arrival < self.window:
                 arrived = 0
@@ -119,10 +127,11 @@ class Server:
                 need = b
             if not group:
                 return
-            g = _Group(group, now)
-            self.used += max(need, g.bytes)
-            g.reserved = max(need, g.bytes)
-            self.active.append(g)
+            # the context is built and its replicas set up (host-side state,
+            # uploads, classification, lists) on a builder thread, while this
+            # loop keeps the other micro-batches' chunks going
+            self.used += need
+            self.building.append((self.pool.submit(_Group, group, now, self.graphs), need))
             self.stats["groups"] += 1
 
     def _collect(self, g, now):
@@ -133,18 +142,38 @@ class Server:
             v = b.vertices(i)
             r.final = {k: v[k] for k in ("x", "y", "z", "vx", "vy", "vz", "ix", "iy", "bondi")}
             r.finished = time.perf_counter() - self.t0
-        b.close()
+        # freeing device memory can wait for the whole device: the builder
+        # thread does it, before any later build allocates
+        self.pool.submit(b.close)
         self.used -= g.reserved
 
     def run(self, requests, idle_sleep=2e-4):
         """Serve `requests` (their arrival times are offsets from now);
         returns them with results and latencies filled in."""
         queue = deque(sorted(requests, key=lambda r: r.arrival))
+        self.pool = cf.ThreadPoolExecutor(max_workers=self.builders)
         self.t0 = time.perf_counter()
-        while queue or self.active:
+        try:
+            self._loop(queue, idle_sleep)
+        finally:
+            self.pool.shutdown(wait=True)
+        self.makespan = time.perf_counter() - self.t0
+        return requests
+
+    def _loop(self, queue, idle_sleep):
+        while queue or self.active or self.building:
             now = time.perf_counter() - self.t0
             self._admit(queue, now)
             progressed = False
+            for item in list(self.building):
+                fut, need = item
+                if fut.done():
+                    g = fut.result()  # a failed build raises here
+                    g.reserved = max(need, g.bytes)
+                    self.used += g.reserved - need
+                    self.active.append(g)
+                    self.building.remove(item)
+                    progressed = True
             for g in list(self.active):
                 if not g.batch.done():
                     continue
@@ -162,8 +191,6 @@ class Server:
                                                      sum(len(g.reqs) for g in self.active))
             if not progressed:
                 time.sleep(idle_sleep)
-        self.makespan = time.perf_counter() - self.t0
-        return requests
 
 
 def poisson_requests(n, rate, steps, seed=0, series_every=100, pool=None):

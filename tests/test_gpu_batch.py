@@ -310,3 +310,36 @@ def test_membrane_chain_of_a_replica_stays_on_one_xcd():
                 assert g["chain_xcd"] == r % 8, (oneq, r, g["chain_xcd"])
         finally:
             b.close()
+
+
+@pytest.mark.gpu
+def test_lists_at_collisions_never_change_results():
+    """Cell->triangle lists updated at the last collision before each
+    replica's deadline (IRBC_OPT_ALIGN_LISTS) give the same bits as the
+    reference's cadence, over 150 steps of a batch whose replicas have list
+    cadences of 6 (shorter than a collision period), 12, 31 and 83 steps;
+    with alignment, list updates happen on collision steps only."""
+    ds = [en.replica(12, 12, 12, 10, 42, 1.8, 1.0, 42, kon=5.0, koff=1.0, shear=3.0, seed=71),
+          en.replica(16, 12, 10, 10, 162, 2.6, 1.0, 162, kon=200.0, koff=20.0, shear=2.0, seed=72),
+          en.replica(12, 12, 8, 8, 42, 1.8, 1.2, 30, kon=5.0, koff=1.0, shear=1.0, seed=73),
+          en.replica(20, 16, 12, 8, 162, 3.0, 1.3, 100, kon=0.5, koff=0.05, shear=0.25, seed=74)]
+    upl = [en.params(d).uplist for d in ds]
+    assert sorted(upl) == [6, 12, 31, 83], upl
+    snaps = []
+    for align in (0, 1):
+        b = en.batch(ds)
+        b.set_option(ia.OPT_ALIGN_LISTS, align)
+        for s in range(150):
+            b.md_steps(1)
+            for r in range(len(ds)):
+                step, colsteps, upd = b.counters(r)
+                if align and upd == 0 and upl[r] >= 10:
+                    assert colsteps == 0, (r, step)  # a list update on a collision step
+        b.sync()
+        snaps.append([_snapshot(b, r) for r in range(len(ds))])
+        b.close()
+    for r in range(len(ds)):
+        for key in snaps[0][r]:
+            if key == "counters":
+                continue  # the list counters differ by design
+            np.testing.assert_array_equal(snaps[0][r][key], snaps[1][r][key], err_msg="replica %d %s" % (r, key))

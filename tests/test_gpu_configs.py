@@ -183,7 +183,7 @@ def test_poisson_stream_of_uneven_requests():
     """Requests drawn like the uneven batch, Poisson arrivals faster than
     they run, served under a budget that makes some wait; sampled results
     equal the same requests served alone."""
-    reqs = serve.poisson_requests(12, 200.0, 300, seed=5, series_every=50)
+    reqs = serve.poisson_requests(12, 200.0, 2000, seed=5, series_every=100)
     budget = int(0.5 * sum(r.bytes for r in reqs))
     srv = serve.Server(budget, chunk=100, max_active=3, window=0.005)
     srv.run(reqs)
@@ -192,7 +192,7 @@ def test_poisson_stream_of_uneven_requests():
     assert srv.stats["groups"] >= 2
     assert srv.stats["max_active_groups"] >= 2
     for r in (reqs[0], reqs[6], reqs[11]):
-        alone = serve.Server(r.bytes * 2, chunk=300)
+        alone = serve.Server(r.bytes * 2, chunk=2000)
         a = serve.Request(r.id, r.replica, r.steps, r.series_every, 0.0)
         alone.run([a])
         np.testing.assert_array_equal(r.series, a.series, err_msg="request %d series" % r.id)

@@ -208,3 +208,24 @@ def test_bonds_bind_unbind_bit_exact(ia):
         np.testing.assert_array_equal(cg, co)
         np.testing.assert_array_equal(lg, lo)
     assert vo["bondi"].sum() > 0
+
+
+@pytest.mark.gpu
+def test_binned_classification_equals_every_triangle_at_full_size(ia):
+    """The binned inside/outside flags equal the all-triangles loop on the
+    64x64x48 replica and on a 2562-vertex stretched membrane (the oracle
+    pins the loop on a small box)."""
+    import ensemble as en
+    for d in (en.single(), en.replica(48, 48, 40, 8, 2562, None, 1.2, 1500, seed=4242)):
+        b = ia.Batch([en.spec(d)])
+        try:
+            en.load(b, 0, d)
+            b.classify()
+            binned = b.solvent(0)["indio"]
+            b.set_option(ia.OPT_CLASSIFY_ALL, 1)
+            b.classify()
+            every = b.solvent(0)["indio"]
+        finally:
+            b.close()
+        assert np.array_equal(binned, every)
+        assert 0 < int((every == 0).sum()) < len(every)  # some inside, most outside
