@@ -174,31 +174,65 @@ def batched(steps, warmup, options=-1):
             "launch_options_index": {"batched": kb, "plain": ka}}
 
 
-def served(n, rate, steps, budget_gb, plain_n=3):
+def plain_rate(ds, steps, warmup=50):
+    """Replica-steps/s of the plain path (one replica per context, the
+    library's default launch shape) over the replicas ds, run one after
+    another: total steps over total time."""
+    t, n = 0.0, 0
+    for d in ds:
+        a = en.batch([d])
+        a.md_steps(warmup)
+        a.sync()
+        t0 = time.perf_counter()
+        a.md_steps(steps)
+        a.sync()
+        t += time.perf_counter() - t0
+        n += steps
+        a.close()
+    return n / t
+
+
+def served(n, rate, steps, budget_gb, plain_every=8, plain_steps=1000):
     """Poisson arrivals of uneven-ensemble requests through the request
-    loop, against the same requests run one at a time."""
+    loop, against the plain path on every plain_every-th request (a sample
+    of the same mix, run one after another)."""
     reqs = serve.poisson_requests(n, rate, steps, seed=1)
-    srv = serve.Server(int(budget_gb * 2 ** 30), chunk=250, max_active=4, window=0.1)
+    srv = serve.Server(int(budget_gb * 2 ** 30), chunk=500, max_active=3, window=0.2)
     srv.run(reqs)
     out = serve.summary(reqs, srv.makespan)
-    t_plain, s_plain = 0.0, 0
-    for r in reqs[:plain_n]:
-        a = en.batch([r.replica])
-        t0 = time.perf_counter()
-        a.md_steps(r.steps)
-        a.sync()
-        t_plain += time.perf_counter() - t0
-        s_plain += r.steps
-        a.close()
+    sample = [r.replica for r in reqs[::plain_every]]
     out.update({"workload": "%d requests, Poisson arrivals at %.1f/s, %d steps each, boxes 24^3..64x64x48 "
                             "at density 5-10, 162/642/2562-vertex ellipsoids" % (n, rate, steps),
                 "hbm_budget_GiB": budget_gb, "chunk_steps": srv.chunk, "batching_window_s": srv.window,
                 "micro_batches": srv.stats["groups"], "max_concurrent_micro_batches": srv.stats["max_active_groups"],
                 "max_resident_replicas": srv.stats["max_resident_replicas"],
-                "plain_replica_steps_per_s": s_plain / t_plain,
-                "plain_sample": "first %d requests run alone, one after another" % plain_n})
+                "plain_replica_steps_per_s": plain_rate(sample, plain_steps),
+                "plain_sample": "every %dth request alone (%d of them), %d timed steps each"
+                                % (plain_every, len(sample), plain_steps)})
     out["speedup_vs_plain"] = out["replica_steps_per_s"] / out["plain_replica_steps_per_s"]
     return out
+
+
+def uneven(steps, plain_every=6, plain_steps=1000):
+    """48 replicas of mixed boxes, meshes, rates and shears in one batch
+    (list updates at collision steps), against the plain path on every
+    plain_every-th of them."""
+    ds = en.uneven()
+    b = en.batch(ds)
+    b.set_option(ia.OPT_ALIGN_LISTS, 1)
+    b.set_option(ia.OPT_GRAPHS, 0)
+    dt = timed(b, steps, 50)
+    b.close()
+    ns = sum(en.n_solvent(d) for d in ds)
+    rb = len(ds) * steps / dt
+    ra = plain_rate(ds[::plain_every], plain_steps)
+    return {"workload": "48 replicas, boxes 24^3..64x64x48 at density 5-10 (%d particles), 162/642/2562-vertex "
+                        "ellipsoids, 100-2000 knobs, shear 0.01-0.1" % ns,
+            "steps": steps, "batched_ms_per_step": dt * 1e3 / steps, "batched_replica_steps_per_s": rb,
+            "batched_particle_steps_per_s": ns * steps / dt, "plain_replica_steps_per_s": ra,
+            "plain_sample": "every %dth replica alone (%d of them), %d timed steps each"
+                            % (plain_every, len(ds[::plain_every]), plain_steps),
+            "speedup": rb / ra}
 
 
 def trace_summary(path, skip, steps):
@@ -249,6 +283,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=2000)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--uneven-steps", type=int, default=1000)
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the 64-replica batch against its first replica alone (for profiling)")
     ap.add_argument("--served-requests", type=int, default=64)
@@ -339,6 +374,8 @@ def main():
     b.close()
     if not a.no_batched:
         line["batched"] = batched(a.batched_steps, 100)
+    if not a.no_batched:
+        line["uneven"] = uneven(a.uneven_steps)
     if not a.no_served:
         line["served"] = served(a.served_requests, a.served_rate, a.served_steps, 16.0)
     print(json.dumps(line))

@@ -91,6 +91,14 @@ int irbc_get_vertices(irbc_ctx *ctx, int r, double *x, double *y, double *z, dou
 int irbc_set_solvent(irbc_ctx *ctx, int r, int64_t n, const double *x, const double *y,
                      const double *z, const double *vx, const double *vy, const double *vz,
                      const int32_t *indio);
+/* a fresh solvent drawn on the device from the replica's counter-based
+ * streams (keyed by its seed): n_solvent positions uniform in the box,
+ * velocities Maxwellian at kT = Theta (maxwell = 1) or uniform in
+ * +-sqrt(3 kT / m) as maincode.cpp:96-107 draws them (maxwell = 0), stored
+ * in cell order; inside flags all outside until irbc_classify_solvent.  No
+ * counterpart in the reference, whose MT19937 draws irbc_set_solvent takes
+ * from the host; for set-up without a host round trip (request loops). */
+int irbc_init_solvent(irbc_ctx *ctx, int r, int maxwell);
 /* positions/velocities as stored (FP32, widened), last shifted-cell key and
  * inside flag, all in particle order; any pointer may be NULL */
 int irbc_get_solvent(irbc_ctx *ctx, int r, double *x, double *y, double *z, double *vx,

@@ -917,6 +917,30 @@ int irbc_set_solvent(irbc_ctx *c, int r, int64_t n, const double *x, const doubl
     return c->push(r);
 }
 
+static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t *out, int only_rep = -1);
+
+int irbc_init_solvent(irbc_ctx *c, int r, int maxwell)
+{
+    CHECK_R(c, r);
+    int rc = c->pull(r);
+    if (rc) return rc;
+    DevRep &R = c->rep[r];
+    const unsigned nb = (unsigned)((R.np + IRBC_WG - 1) / IRBC_WG), T = (unsigned)c->z.tiles;
+    k_init_solvent<<<nb, IRBC_WG, 0, c->st>>>(c->d_rep, r, maxwell ? 1 : 0);
+    k_init_bin<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, r, c->skey, c->srank, c->ccount);
+    rc = launch_scan(c, 1, 0, c->ccount, c->cstart, r);
+    if (rc) return rc;
+    k_scatter<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, 0, 1, c->skey, c->srank, c->cstart, r);
+    k_unpack<<<nb, IRBC_WG, 0, c->st>>>(c->d_rep, r);
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(c->st));
+    rc = c->pull(r); /* the scan epoch the binning advanced */
+    if (rc) return rc;
+    c->rep[r].rotate_pending = 0;
+    c->hr[r].solvent = true;
+    return c->push(r);
+}
+
 int irbc_get_solvent(irbc_ctx *c, int r, double *x, double *y, double *z, double *vx, double *vy,
                      double *vz, int32_t *key, int32_t *indio)
 {
@@ -1144,11 +1168,11 @@ int irbc_set_bond_tape(irbc_ctx *c, int r, const double *u, int64_t n)
 
 /* ---------------- stages ---------------- */
 
-static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t *out)
+static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t *out, int only_rep)
 {
     const unsigned nb = (unsigned)c->z.scanblk;
     k_scan_lb<<<nb, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_blk, (int)nb, staged, which, in, out, c->scanflag,
-                                         c->scanticket);
+                                         c->scanticket, only_rep);
     LAUNCHCHK();
     return IRBC_OK;
 }
@@ -1179,7 +1203,7 @@ static int launch_collision(irbc_ctx *c, int staged, int diag)
     int rc = launch_scan(c, staged, 0, c->ccount, c->cstart);
     if (rc) return rc;
     k_scatter<<<xc ? xcd_grid(T) : T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, xc, staged, c->skey,
-                                                           c->srank, c->cstart);
+                                                           c->srank, c->cstart, -1);
     k_cell<<<xc ? xcd_grid(CT) : CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, xc, staged, c->cstart,
                                                          c->vhead,
                                                 c->vnext,

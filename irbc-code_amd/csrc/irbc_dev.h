@@ -42,6 +42,8 @@
 #define IRBC_RS_WALL 3u
 #define IRBC_RS_MEM 4u
 #define IRBC_RS_BOND 5u
+#define IRBC_RS_INIT 6u     /* a fresh solvent's positions (irbc_init_solvent) */
+#define IRBC_RS_INITV 7u    /* and velocities */
 
 #define IRBC_SERIES_REC 8      /* doubles per time-series record */
 /* lanes per vertex in the membrane chain's gather.  More lanes fetch a

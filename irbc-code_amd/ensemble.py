@@ -94,22 +94,28 @@ def spec(d):
     return ia.make_spec(params(d), n_solvent(d), len(V), len(E), len(T), seed=d["seed"])
 
 
-def state(d):
-    """The replica's initial state as host arrays."""
+def state(d, host_solvent=True):
+    """The replica's initial state as host arrays (the solvent drawn on the
+    host unless host_solvent is False)."""
     p = params(d)
     V, E, T = mesh(d)
     return dict(V=V, E=E, T=T,
-                sol=sc.solvent(n_solvent(d), d["Lx"], d["Ly"], d["Lz"], d["seed"]),
+                sol=sc.solvent(n_solvent(d), d["Lx"], d["Ly"], d["Lz"], d["seed"]) if host_solvent else None,
                 vel=sc.vertex_velocities(d["nv"], d["seed"] + 1, pmass=p.pmass),
                 knob=sc.knobs_uniform(d["nv"], d["nknobs"], d["seed"] + 2),
                 para=sc.parasite_state(V, d["seed"] + 3))
 
 
-def load(b, r, d, st=None):
-    """Put replica d's initial state into slot r of batch b."""
-    st = state(d) if st is None else st
+def load(b, r, d, st=None, device_solvent=False):
+    """Put replica d's initial state into slot r of batch b; with
+    device_solvent the solvent is drawn on the device from the replica's
+    seed instead of on the host (no host arrays, no upload)."""
+    st = state(d, host_solvent=not device_solvent) if st is None else st
     b.set_mesh(r, st["V"], st["E"], st["T"])
-    b.set_solvent(r, *st["sol"])
+    if device_solvent:
+        b.init_solvent(r)
+    else:
+        b.set_solvent(r, *st["sol"])
     v = st["vel"]
     b.set_vertices(r, vx=v[:, 0], vy=v[:, 1], vz=v[:, 2], knob=st["knob"])
     b.set_parasite(r, *st["para"])

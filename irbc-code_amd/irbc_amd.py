@@ -27,7 +27,7 @@ EXPORTS = (
     "irbc_get_pairs", "irbc_md_steps", "irbc_set_diagnostics", "irbc_profile",
     "irbc_profile_read", "irbc_set_series", "irbc_get_series", "irbc_query", "irbc_set_option",
     "irbc_checkpoint_bytes", "irbc_checkpoint_save", "irbc_checkpoint_load", "irbc_autotune",
-    "irbc_membrane_va_static",
+    "irbc_membrane_va_static", "irbc_init_solvent",
 )
 
 OPT_MEMBRANE_FUSED = 1
@@ -279,6 +279,10 @@ class Batch:
         ind = _i32(indio)
         _chk(self.L.irbc_set_solvent(self.h, r, len(arrs[0]), *[_ptr(a, _d) for a in arrs],
                                      _ptr(ind, _i)))
+
+    def init_solvent(self, r, maxwell=True):
+        """A fresh solvent drawn on the device from the replica's seed."""
+        _chk(self.L.irbc_init_solvent(self.h, r, 1 if maxwell else 0))
 
     def solvent(self, r=0):
         n = self.specs[r].n_solvent

@@ -69,8 +69,11 @@ class _Group:
         # its chunks take many shapes: captured graphs would rarely be
         # replayed
         self.batch.set_option(ia.OPT_GRAPHS, 1 if graphs else 0)
+        # list updates (and the near-mask rebuilds after them) beside the
+        # collisions, whatever each replica's cadence
+        self.batch.set_option(ia.OPT_ALIGN_LISTS, 1)
         for i, r in enumerate(reqs):
-            en.load(self.batch, i, r.replica)
+            en.load(self.batch, i, r.replica, device_solvent=True)
             if r.series_every > 0:
                 self.batch.set_series(i, r.series_every, max(1, r.steps // r.series_every))
         en.start(self.batch)

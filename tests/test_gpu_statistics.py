@@ -52,3 +52,39 @@ def test_shear_profile_linear_and_kT_one():
     assert abs(slope - w / Lz) < 0.15 * w / Lz, (slope, w / Lz)
     kT = t_sum / t_n
     assert abs(kT - 1.0) < 0.05, kT
+
+
+@pytest.mark.gpu
+def test_device_drawn_solvent():
+    """irbc_init_solvent: the same seed draws the same solvent in any slot
+    of any batch, another seed another one; positions lie in the box and
+    velocities have kT = 1 (Maxwellian) or the reference's uniform spread
+    (all read back in particle order, through the ids)."""
+    import ensemble as en
+    import irbc_amd as ia
+    d = en.replica(20, 16, 12, 10, 162, 2.6, 1.0, 100, seed=91)
+    e = en.replica(16, 12, 10, 10, 42, 1.8, 1.0, 20, seed=92)
+    a = ia.Batch([en.spec(d)])
+    b = ia.Batch([en.spec(e), en.spec(d)])
+    try:
+        for bb, r, dd in ((a, 0, d), (b, 0, e), (b, 1, d)):
+            en.load(bb, r, dd, device_solvent=True)
+        sa, sb, se = a.solvent(0), b.solvent(1), b.solvent(0)
+        for k in ("x", "y", "z", "vx", "vy", "vz"):
+            np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+        assert not np.array_equal(sa["vx"][:len(se["vx"])], se["vx"])
+        n = len(sa["x"])
+        assert n == 20 * 16 * 12 * 10
+        assert sa["x"].min() >= 0 and sa["x"].max() < 20 and sa["y"].max() < 16
+        assert sa["z"].min() >= 0 and sa["z"].max() <= 12
+        v = np.concatenate([sa["vx"], sa["vy"], sa["vz"]])
+        assert abs(v.mean()) < 0.01 and abs(v.var() - 1.0) < 0.01
+        assert abs(sa["x"].mean() - 10.0) < 0.1
+        b.init_solvent(1, maxwell=False)
+        u = b.solvent(1)
+        assert np.abs(u["vx"]).max() <= np.sqrt(3.0) and abs(np.var(u["vx"]) - 1.0) < 0.02
+        a.md_steps(20)
+        a.sync()
+    finally:
+        a.close()
+        b.close()
