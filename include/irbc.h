@@ -137,11 +137,19 @@ int irbc_classify_solvent(irbc_ctx *ctx);
 #define IRBC_PH_KICK1 1
 #define IRBC_PH_DRIFT 2
 #define IRBC_PH_VA 4
-#define IRBC_PH_FORCES 8
+#define IRBC_PH_FORCES 8 /* IRBC_PH_PARASITE then IRBC_PH_MEMBRANE, one pass */
 #define IRBC_PH_EXTERNAL 16
 #define IRBC_PH_BONDS 32
 #define IRBC_PH_KICK2 64
 #define IRBC_PH_COUNTERS 128
+/* pfdt() alone (Src/memfuncs.cpp:443-484, Include/memfuncs.h:28): the
+ * bead-membrane repulsion, subtracted from the vertex forces and added to
+ * the bead's */
+#define IRBC_PH_PARASITE 512
+/* force_calculate_dt() alone (Include/memfuncs.h:27): springs, bending,
+ * area and volume terms added to the vertex forces, with the area and
+ * volume of the last IRBC_PH_VA */
+#define IRBC_PH_MEMBRANE 1024
 int irbc_membrane_phases(irbc_ctx *ctx, int phases);
 int irbc_membrane_va(irbc_ctx *ctx, int r, double *area, double *volume);
 int irbc_membrane_forces(irbc_ctx *ctx);

@@ -35,7 +35,10 @@ def run(cmd, cwd=None):
 def build_lib():
     out = os.path.join(HERE, "libirbc_amd.so")
     tmp = out + ".tmp"
-    run([HIPCC] + HIPFLAGS + ["-shared", "-o", tmp, os.path.join(HERE, "csrc", "irbc.hip")])
+    # the soname lets a process that loaded this file by path and then
+    # libirbc_host.so (which needs it by name) share one copy
+    run([HIPCC] + HIPFLAGS + ["-shared", "-Wl,-soname,libirbc_amd.so", "-o", tmp,
+                              os.path.join(HERE, "csrc", "irbc.hip")])
     os.replace(tmp, out)
     return out
 

@@ -1550,10 +1550,15 @@ int irbc_classify_solvent(irbc_ctx *c)
 int irbc_membrane_phases(irbc_ctx *c, int phases)
 {
     if (!c) return fail(IRBC_ERR_ARG, "null context");
+    if (phases & ~(255 | IRBC_PH_PARASITE | IRBC_PH_MEMBRANE)) return fail(IRBC_ERR_ARG, "unknown phase bits 0x%x", phases);
+    /* the public force bits onto the kernels' two force phases */
+    int dev = phases & 255 & ~PH_FORCES;
+    if (phases & (IRBC_PH_FORCES | IRBC_PH_MEMBRANE)) dev |= PH_FORCES;
+    if (phases & (IRBC_PH_FORCES | IRBC_PH_PARASITE)) dev |= PH_WCA;
     c->primed = false;
     int rc = ready(c, false);
     if (rc) return rc;
-    rc = launch_membrane(c, phases, c->st);
+    rc = launch_membrane(c, dev, c->st);
     if (rc) return rc;
     if (phases & PH_COUNTERS) {
         for (int r = 0; r < c->nrep; ++r) advance_host(c->hr[r].spec.params, c->hr[r], c->align_lists);

@@ -38,14 +38,12 @@ void streamstep() { check(irbc_stream(ctx()), "streamstep"); }
 
 void TotalVA() { check(irbc_membrane_va(ctx(), g_rep, &Sarea, &Volume), "TotalVA"); }
 
-/* membrane forces with the area/volume of the last TotalVA(), as in the
- * reference; the parasite's WCA forces are computed here too */
-void force_calculate_dt() { check(irbc_membrane_phases(ctx(), IRBC_PH_FORCES), "force_calculate_dt"); }
+/* the membrane's forces with the area/volume of the last TotalVA(), added
+ * to the vertex forces, as in the reference */
+void force_calculate_dt() { check(irbc_membrane_phases(ctx(), IRBC_PH_MEMBRANE), "force_calculate_dt"); }
 
-/* the reference calls pfdt() right before force_calculate_dt()
- * (maincode.cpp:602-604); here both force sets come from one pass, in
- * force_calculate_dt(), so this call has nothing left to do */
-void pfdt() {}
+/* the bead-membrane repulsion: vertex forces and the bead's */
+void pfdt() { check(irbc_membrane_phases(ctx(), IRBC_PH_PARASITE), "pfdt"); }
 
 void externalforces() { check(irbc_membrane_phases(ctx(), IRBC_PH_EXTERNAL), "externalforces"); }
 

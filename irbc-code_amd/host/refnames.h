@@ -14,8 +14,9 @@
 
 #include "irbc.h"
 
-/* the context and replica the reference-named functions act on */
-void irbc_ref_bind(irbc_ctx *ctx, int replica);
+/* the context and replica the reference-named functions act on (C
+ * linkage, so an FFI can bind a handle it created) */
+extern "C" void irbc_ref_bind(irbc_ctx *ctx, int replica);
 
 /* Src/solfuncs.cpp */
 void CollisionStep();        /* Include/solfuncs.h:21 */
@@ -26,8 +27,8 @@ void streamstep();
 
 /* Src/memfuncs.cpp: results land in the globals below, as in the reference */
 void TotalVA();              /* Include/memfuncs.h:25 */
-void force_calculate_dt();   /* Include/memfuncs.h:27 (includes the work of pfdt) */
-void pfdt();                 /* Include/memfuncs.h:28 (no work of its own, see refnames.cpp) */
+void force_calculate_dt();   /* Include/memfuncs.h:27 */
+void pfdt();                 /* Include/memfuncs.h:28 */
 /* gravity, bond springs and wall repulsion, maincode.cpp:606-665 */
 void externalforces();
 

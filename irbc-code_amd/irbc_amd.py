@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libirbc_amd.so")
 PH_KICK1, PH_DRIFT, PH_VA, PH_FORCES = 1, 2, 4, 8
 PH_EXTERNAL, PH_BONDS, PH_KICK2, PH_COUNTERS = 16, 32, 64, 128
 PH_ALL = 255
+PH_PARASITE, PH_MEMBRANE = 512, 1024  # the two halves of PH_FORCES (pfdt, force_calculate_dt)
 
 EXPORTS = (
     "irbc_create", "irbc_destroy", "irbc_last_error", "irbc_device_bytes", "irbc_estimate_bytes",

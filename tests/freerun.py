@@ -88,9 +88,9 @@ def readme_state(seed=12345, kon=0.005, koff=0.015, knobs=500, ns=960000):
     return o, Vp, E, T
 
 
-def run(b, o, n, chunks=(None,)):
+def run(b, o, n, chunks=(None,), each=None):
     """n steps on both sides; the device's in the given call sizes (None:
-    one call for the rest)."""
+    one call for the rest); each(k) is called after the oracle's k-th step."""
     done = 0
     for c in chunks:
         k = n - done if c is None else min(c, n - done)
@@ -100,8 +100,10 @@ def run(b, o, n, chunks=(None,)):
     if done < n:
         b.md_steps(n - done)
     b.sync()
-    for _ in range(n):
+    for k in range(n):
         o.md_step()
+        if each:
+            each(k + 1)
 
 
 def compare(b, r, o, cells=False):
@@ -149,3 +151,38 @@ def compare(b, r, o, cells=False):
     out["parasite_rel"] = float(np.abs(pg[0][:6] - pog[0][:6]).max() / max(1e-300, np.abs(pog[0][:6]).max()))
     out["counters_equal"] = tuple(b.counters(r)) == tuple(o.counters())
     return out
+
+
+def series_record(o):
+    """The time-series record of the oracle's state after a step: step,
+    membrane centroid (unwrapped), centroid velocity, bonded knobs."""
+    v = o.vertices()
+    L = (o.p.Lx, o.p.Ly)
+    return np.array([o.counters()[0], np.mean(v["x"] + v["ix"] * L[0]), np.mean(v["y"] + v["iy"] * L[1]),
+                     np.mean(v["z"]), np.mean(v["vx"]), np.mean(v["vy"]), np.mean(v["vz"]),
+                     np.count_nonzero(v["bondi"] == 1)])
+
+
+def check(m, L, what):
+    """The tolerances of a free-running comparison (FP32 solvent storage on
+    both sides, FP64 arithmetic; the oracle's sequential sums against the
+    device's exact fixed-point ones, glibc against the device's libm):
+    exact keys, counts, flags, counters, images, bonds and ligands; at most
+    1% of the particles off by at most 8 FP32 units at the box scale or
+    1e-5 in velocity; the membrane within 1e-6 cells and 1e-5 of its
+    velocity and force scales; area, volume and bead within 1e-8."""
+    assert m["counters_equal"], what
+    assert m["key_diff"] == 0, (what, m)
+    assert m["indio_diff"] == 0, (what, m)
+    if "nc_diff" in m:
+        assert m["nc_diff"] == 0, (what, m)
+        assert m["vcm_rel"] <= 1e-5, (what, m)
+    assert m["solvent_diff"] <= 0.01 * m["n"], (what, m)
+    assert m["solvent_pos_max"] <= 8 * np.spacing(np.float32(max(L))), (what, m)
+    assert m["solvent_vel_max"] <= 1e-5, (what, m)
+    assert m["mem_pos_max"] <= 1e-6, (what, m)
+    assert m["mem_vel_rel"] <= 1e-5, (what, m)
+    assert m["mem_force_rel"] <= 1e-5, (what, m)
+    assert m["area_rel"] <= 1e-8 and m["volume_rel"] <= 1e-8, (what, m)
+    assert m["parasite_rel"] <= 1e-8, (what, m)
+    assert m["bonds_equal"] and m["images_equal"] and m["ligands_equal"], (what, m)

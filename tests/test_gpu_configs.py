@@ -8,6 +8,8 @@ device does; see freerun.py):
     reference builds it (reproduced bit for bit by the oracle against the
     reference's own fixtures), 90 steps: 9 collisions and a list update;
     the same with fast bond kinetics (kon = koff = 40), in three calls;
+    the time series of both runs (centroid, centroid velocity, bonded
+    knobs) against the same quantities of the oracle's state;
   - the single large replica: 64x64x48 (1 966 080 particles), 642-vertex
     icosphere, 30 steps with a collision and a list update;
   - a 2562-vertex stretched membrane in a 48x48x40 box at density 8;
@@ -44,37 +46,31 @@ def _threads():
     po.set_threads(16)
 
 
-def check(m, L, what):
-    assert m["counters_equal"], what
-    assert m["key_diff"] == 0, (what, m)
-    assert m["indio_diff"] == 0, (what, m)
-    if "nc_diff" in m:
-        assert m["nc_diff"] == 0, (what, m)
-        assert m["vcm_rel"] <= 1e-5, (what, m)
-    assert m["solvent_diff"] <= 0.01 * m["n"], (what, m)
-    assert m["solvent_pos_max"] <= 8 * np.spacing(np.float32(max(L))), (what, m)
-    assert m["solvent_vel_max"] <= 1e-5, (what, m)
-    assert m["mem_pos_max"] <= 1e-6, (what, m)
-    assert m["mem_vel_rel"] <= 1e-5, (what, m)
-    assert m["mem_force_rel"] <= 1e-5, (what, m)
-    assert m["area_rel"] <= 1e-8 and m["volume_rel"] <= 1e-8, (what, m)
-    assert m["parasite_rel"] <= 1e-8, (what, m)
-    assert m["bonds_equal"] and m["images_equal"] and m["ligands_equal"], (what, m)
-
-
-def _readme(seed, kon, koff, n, chunks):
+def _readme(seed, kon, koff, n, chunks, every=10):
     o, V, E, T = fr.readme_state(seed, kon, koff)
     o.set_fp32(True)
     o.use_philox(seed)
     b = fr.device_from_oracle(o, V, E, T, seed)
+    want = []
     try:
         b.set_diagnostics(True)
-        fr.run(b, o, n, chunks)
+        b.set_series(0, every, n // every)
+        fr.run(b, o, n, chunks, each=lambda k: want.append(fr.series_record(o)) if k % every == 0 else None)
         m = fr.compare(b, 0, o, cells=True)
+        got = b.series(0)
     finally:
         b.close()
     print("README seed %d kon %g koff %g, %d steps:" % (seed, kon, koff, n), m)
-    check(m, (o.p.Lx, o.p.Ly, o.p.Lz), "README")
+    fr.check(m, (o.p.Lx, o.p.Ly, o.p.Lz), "README")
+    # the time series (centroid, centroid velocity, bonded knobs every
+    # `every` steps) against the same quantities of the oracle's state
+    want = np.array(want)
+    assert got.shape == want.shape, (got.shape, want.shape)
+    np.testing.assert_array_equal(got[:, 0], want[:, 0])
+    np.testing.assert_array_equal(got[:, 7], want[:, 7])
+    np.testing.assert_allclose(got[:, 1:4], want[:, 1:4], rtol=0, atol=1e-6)
+    vscale = np.abs(o.vertices()["vx"]).max()
+    np.testing.assert_allclose(got[:, 4:7], want[:, 4:7], rtol=0, atol=1e-5 * vscale)
     return m
 
 
@@ -104,7 +100,7 @@ def _single(d, n, upd_before):
     finally:
         b.close()
     print(d, m)
-    check(m, (d["Lx"], d["Ly"], d["Lz"]), "replica")
+    fr.check(m, (d["Lx"], d["Ly"], d["Lz"]), "replica")
     return m
 
 
@@ -141,7 +137,7 @@ def _batch_case(ds, sample_oracle, sample_alone, n_oracle=20, n_total=120):
                 o.md_step()
             m = fr.compare(b, r, o)
             print("replica", r, ds[r], m)
-            check(m, (ds[r]["Lx"], ds[r]["Ly"], ds[r]["Lz"]), "replica %d in batch" % r)
+            fr.check(m, (ds[r]["Lx"], ds[r]["Ly"], ds[r]["Lz"]), "replica %d in batch" % r)
         del os_
         b.md_steps(n_total - n_oracle)
         b.sync()
