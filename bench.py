@@ -286,6 +286,8 @@ def main():
     ap.add_argument("--uneven-steps", type=int, default=1000)
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the 64-replica batch against its first replica alone (for profiling)")
+    ap.add_argument("--only", choices=("batched", "uneven", "served"),
+                    help="run only that leg and print its JSON object (for profiling)")
     ap.add_argument("--served-requests", type=int, default=64)
     ap.add_argument("--served-rate", type=float, default=32.0)
     ap.add_argument("--served-steps", type=int, default=5000)
@@ -305,8 +307,14 @@ def main():
               % a.gpus, file=sys.stderr)
         sys.exit(2)
     ia.load()
-    if a.batched_only:
+    if a.batched_only or a.only == "batched":
         print(json.dumps({"batched": batched(a.batched_steps, 100, a.options)}))
+        return
+    if a.only == "uneven":
+        print(json.dumps({"uneven": uneven(a.uneven_steps)}))
+        return
+    if a.only == "served":
+        print(json.dumps({"served": served(a.served_requests, a.served_rate, a.served_steps, 16.0)}))
         return
     d = en.single()
     b = ia.Batch([en.spec(d)])

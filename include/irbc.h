@@ -65,6 +65,9 @@ typedef struct irbc_replica_spec {
 /* ---- context ---- */
 int irbc_create(const irbc_replica_spec *specs, int n_replicas, int device, irbc_ctx **out);
 int irbc_destroy(irbc_ctx *ctx);
+/* replica r's spec as created: sizes, parameters, seed (the box and mesh
+ * sizes the reference keeps in define.h:42-45 and its mesh files) */
+int irbc_get_spec(const irbc_ctx *ctx, int r, irbc_replica_spec *out);
 const char *irbc_last_error(void);
 int irbc_device_bytes(const irbc_ctx *ctx, int64_t *bytes);
 /* bytes a batch of these replicas would take on the device (admission) */

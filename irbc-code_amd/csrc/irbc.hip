@@ -1022,6 +1022,14 @@ int irbc_get_ligands(irbc_ctx *c, int r, int32_t *ligi)
     return IRBC_OK;
 }
 
+int irbc_get_spec(const irbc_ctx *c, int r, irbc_replica_spec *out)
+{
+    CHECK_R(c, r);
+    if (!out) return fail(IRBC_ERR_ARG, "irbc_get_spec: null output");
+    *out = c->hr[r].spec;
+    return IRBC_OK;
+}
+
 int irbc_get_globals(irbc_ctx *c, int r, double *g)
 {
     CHECK_R(c, r);

@@ -3,10 +3,13 @@
  */
 #include "refnames.h"
 
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <vector>
 
-double Sarea = 0.0, Volume = 0.0;
+double Sarea = 0.0, Volume = 0.0, Sarea0 = 0.0, Volume0 = 0.0;
+double center[3] = {0.0, 0.0, 0.0};
 
 static irbc_ctx *g_ctx = nullptr;
 static int g_rep = 0;
@@ -53,4 +56,39 @@ void mdsteps(long n)
 {
     check(irbc_md_steps(ctx(), n), "mdsteps");
     check(irbc_sync(ctx()), "mdsteps");
+}
+
+/* the current shape's area and volume as the rest values: the reference
+ * calls it once, on the set-up mesh (maincode.cpp:344), which is where
+ * irbc_set_mesh takes the device's own rest values from */
+void TotalVA0() { check(irbc_membrane_va(ctx(), g_rep, &Sarea0, &Volume0), "TotalVA0"); }
+
+/* the membrane centroid, folded into the box (Src/memfuncs.cpp:55-74) */
+void centerofmass()
+{
+    irbc_replica_spec s;
+    check(irbc_get_spec(ctx(), g_rep, &s), "centerofmass");
+    const int nv = s.n_vertices;
+    std::vector<double> x(nv), y(nv), z(nv);
+    std::vector<int32_t> ix(nv), iy(nv);
+    check(irbc_get_vertices(ctx(), g_rep, x.data(), y.data(), z.data(), nullptr, nullptr, nullptr, nullptr, nullptr,
+                            nullptr, nullptr, nullptr, ix.data(), iy.data(), nullptr, nullptr, nullptr, nullptr),
+          "centerofmass");
+    const double L[3] = {(double)s.params.Lx, (double)s.params.Ly, (double)s.params.Lz};
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int i = 0; i < nv; ++i) {
+        a[0] += x[i] + ix[i] * L[0];
+        a[1] += y[i] + iy[i] * L[1];
+        a[2] += z[i];
+    }
+    for (int k = 0; k < 3; ++k) center[k] = std::fmod(a[k] / (double)nv, L[k]);
+}
+
+/* a solvent coordinate wrapped into [0, lx] by whole box lengths
+ * (Src/solfuncs.cpp:603-622; the device wraps inside its streaming) */
+double bbound(double x, double lx)
+{
+    while (x > lx) x -= lx;
+    while (x < 0) x += lx;
+    return x;
 }

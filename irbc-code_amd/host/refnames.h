@@ -38,6 +38,14 @@ void bond_dynamics();
 /* the time loop of maincode.cpp:461-764 for n steps */
 void mdsteps(long n);
 
-extern double Sarea, Volume; /* set by TotalVA() (define.h:391) */
+/* Src/memfuncs.cpp: TotalVA0 (Include/memfuncs.h:26), centerofmass (:20) */
+void TotalVA0();
+void centerofmass();
+/* Src/solfuncs.cpp:603-622 (Include/solfuncs.h:25) */
+double bbound(double x, double lx);
+
+extern double Sarea, Volume;   /* set by TotalVA() (define.h:391) */
+extern double Sarea0, Volume0; /* set by TotalVA0() (define.h:338) */
+extern double center[3];       /* set by centerofmass() (define.h:281) */
 
 #endif

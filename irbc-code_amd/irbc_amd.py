@@ -28,7 +28,7 @@ EXPORTS = (
     "irbc_get_pairs", "irbc_md_steps", "irbc_set_diagnostics", "irbc_profile",
     "irbc_profile_read", "irbc_set_series", "irbc_get_series", "irbc_query", "irbc_set_option",
     "irbc_checkpoint_bytes", "irbc_checkpoint_save", "irbc_checkpoint_load", "irbc_autotune",
-    "irbc_membrane_va_static", "irbc_init_solvent",
+    "irbc_membrane_va_static", "irbc_init_solvent", "irbc_get_spec",
 )
 
 OPT_MEMBRANE_FUSED = 1

@@ -47,7 +47,8 @@ HOST = os.path.join(os.path.dirname(HERE), "irbc-code_amd", "libirbc_host.so")
 REF = {"CollisionStep": "_Z13CollisionStepv", "llistupdate": "_Z11llistupdatev",
        "streamstep": "_Z10streamstepv", "TotalVA": "_Z7TotalVAv", "pfdt": "_Z4pfdtv",
        "force_calculate_dt": "_Z18force_calculate_dtv", "externalforces": "_Z14externalforcesv",
-       "bond_dynamics": "_Z13bond_dynamicsv", "sortsolvent_initial1": "_Z20sortsolvent_initial1v"}
+       "bond_dynamics": "_Z13bond_dynamicsv", "sortsolvent_initial1": "_Z20sortsolvent_initial1v",
+       "TotalVA0": "_Z8TotalVA0v", "centerofmass": "_Z12centerofmassv"}
 
 
 class RefNames:
@@ -73,6 +74,18 @@ class RefNames:
     @property
     def Volume(self):
         return C.c_double.in_dll(self.L, "Volume").value
+
+    @property
+    def Sarea0(self):
+        return C.c_double.in_dll(self.L, "Sarea0").value
+
+    @property
+    def Volume0(self):
+        return C.c_double.in_dll(self.L, "Volume0").value
+
+    @property
+    def center(self):
+        return np.array((C.c_double * 3).in_dll(self.L, "center"))
 
 
 def _partial_step(o, p, dev=None, h=None, b=None):
@@ -267,6 +280,13 @@ def test_pfdt_and_force_calculate_dt_as_two_calls():
         go = o.globals()
         assert abs(h.Sarea - go["Sarea"]) <= 1e-12 * go["Sarea"]
         assert abs(h.Volume - go["Volume"]) <= 1e-12 * go["Volume"]
+        # TotalVA0() of the same shape, and centerofmass()
+        h.TotalVA0()
+        assert (h.Sarea0, h.Volume0) == (h.Sarea, h.Volume)
+        o.centerofmass()
+        h.centerofmass()
+        go = o.globals()
+        np.testing.assert_allclose(h.center, [go["cx"], go["cy"], go["cz"]], rtol=1e-12, atol=1e-12)
         o.pfdt()
         h.pfdt()
         vo, vg = o.vertices(), b.vertices()

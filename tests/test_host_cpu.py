@@ -78,9 +78,29 @@ def test_readme_setup_matches_oracle(seed, nknobs):
 def test_reference_names_exported():
     out = subprocess.run(["nm", "-D", "--defined-only", HOST], capture_output=True, text=True, check=True).stdout
     for name in ("_Z13CollisionStepv", "_Z11llistupdatev", "_Z20sortsolvent_initial1v", "_Z7TotalVAv",
-                 "_Z18force_calculate_dtv", "_Z4pfdtv", "_Z13bond_dynamicsv", "irbc_ref_bind",
-                 "irbc_host_readme_scenario"):
+                 "_Z18force_calculate_dtv", "_Z4pfdtv", "_Z13bond_dynamicsv", "_Z8TotalVA0v",
+                 "_Z12centerofmassv", "_Z6bbounddd", "irbc_ref_bind", "irbc_host_readme_scenario",
+                 "Sarea0", "center"):
         assert name in out, name
+
+
+def test_bbound_wraps_by_whole_box_lengths():
+    """bbound (Src/solfuncs.cpp:603-622): [0, lx] by repeated whole-length
+    steps, the ends kept as they are."""
+    L = C.CDLL(HOST)
+    f = L._Z6bbounddd
+    f.argtypes = [C.c_double, C.c_double]
+    f.restype = C.c_double
+    lx = 60.0
+    assert f(lx, lx) == lx and f(0.0, lx) == 0.0 and f(-0.0, lx) == 0.0
+    for x in np.random.default_rng(3).uniform(-3 * lx, 4 * lx, 200):
+        want = x
+        while want > lx:
+            want -= lx
+        while want < 0:
+            want += lx
+        assert f(x, lx) == want
+        assert 0.0 <= f(x, lx) <= lx
 
 
 def test_driver_argument_errors():
