@@ -73,10 +73,29 @@ def cpu_baseline(b, d, st, budget_s=15.0, max_steps=5000):
     dt = time.perf_counter() - t0
     ns = len(s["x"])
     del o
-    return {"value": ns * n / dt, "unit": "particle-steps/s", "cores": int(ncores), "kind": "port",
-            "sample": "FP64 oracle, same 64x64x48 state, %d MD steps (%d collisions), %.1f s"
-                      % (n, n // 10, dt),
-            "accuracy": accuracy(b, pp, st, po)}
+    out = {"value": ns * n / dt, "unit": "particle-steps/s", "cores": int(ncores), "kind": "port",
+           "sample": "FP64 oracle, same 64x64x48 state, %d MD steps (%d collisions), %.1f s"
+                     % (n, n // 10, dt),
+           "accuracy": accuracy(b, pp, st, po)}
+    out["calibration"] = calibration(out["value"], ncores)
+    return out
+
+
+def calibration(oracle_rate, ncores):
+    """The oracle against the reference's own CPU path (its sources, built
+    and timed on the same cores in the build container by
+    oracle/calibrate.py on the README run): the ratio at the nearest thread
+    count, and the reference's rate it implies here."""
+    path = os.path.join(REPO, "oracle", "calibration.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        cal = json.load(f)
+    run = min(cal["runs"], key=lambda r: abs(r["threads"] - ncores))
+    ratio = run["oracle_over_reference"]
+    return {"oracle_over_reference": ratio, "at_threads": run["threads"], "measured_on": cal["cpu"],
+            "workload": cal["workload"], "reference_particle_steps_per_s": run["reference_particle_steps_per_s"],
+            "implied_reference_rate_here": oracle_rate / ratio}
 
 
 def accuracy(b, pp, st, po):
