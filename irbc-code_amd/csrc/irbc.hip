@@ -549,6 +549,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
         R.xa = c->xa; R.ya = c->ya; R.za = c->za; R.vxa = c->vxa; R.vya = c->vya; R.vza = c->vza;
         R.ida = c->ida;
         R.rec = c->rec;
+        R.rec1 = c->rec + 4 * c->z.slots;
         top_of_step(p, 0, -1, 0, R);
         /* ligand grid coordinates (maincode.cpp:77-93) */
         for (double m = 0.0; m < p.Lx; m += p.lig_spacing) lx.push_back(m);
@@ -911,7 +912,8 @@ int irbc_set_solvent(irbc_ctx *c, int r, int64_t n, const double *x, const doubl
     }
     HIPCHK(hipMemcpy(c->ida + R.poff, id.data(), n * 4, hipMemcpyHostToDevice));
     /* no collision yet: the records' keys (the particles' last cell keys) read 0 */
-    HIPCHK(hipMemset(c->rec + 8 * R.poff, 0, (size_t)n * 32));
+    HIPCHK(hipMemset(R.rec + 4 * R.poff, 0, (size_t)n * 16));
+    HIPCHK(hipMemset(R.rec1 + 4 * R.poff, 0, (size_t)n * 16));
     R.rotate_pending = 0;
     c->hr[r].solvent = true;
     return c->push(r);
@@ -2018,7 +2020,8 @@ int irbc_checkpoint_load(irbc_ctx *c, int r, const void *buf, int64_t n)
     HIPCHK(hipMemcpy(c->lloff + R.coff1, off.data(), (H.ncell + 1) * 4, hipMemcpyHostToDevice));
     if (H.llen > 0) HIPCHK(hipMemcpy(c->llent + R.llent_off, p, H.llen * 4, hipMemcpyHostToDevice));
     /* the particle order of the checkpoint, no pending rotation */
-    HIPCHK(hipMemset(c->rec + 8 * R.poff, 0, (size_t)H.ns * 32));
+    HIPCHK(hipMemset(R.rec + 4 * R.poff, 0, (size_t)H.ns * 16));
+    HIPCHK(hipMemset(R.rec1 + 4 * R.poff, 0, (size_t)H.ns * 16));
     R.rotate_pending = 0;
     R.seed = H.seed;
     R.tapes = H.tapes;

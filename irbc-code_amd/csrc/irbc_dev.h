@@ -140,13 +140,14 @@ struct DevRep {
     /* ---- particle buffers (packed arrays, this replica at poff) ---- */
     float *xa, *ya, *za, *vxa, *vya, *vza;
     uint32_t *ida;          /* particle id | inside-flag << 31 */
-    /* The last collision's counting sort: one 32-byte record per slot, in
-     * cell order (vx vy vz, the shifted-cell key as bits, x y z, the id as
-     * bits: k_cell reads only the first 16 bytes).
+    /* The last collision's counting sort: per slot, in cell order, a
+     * 16-byte velocity record in one plane (vx vy vz, the shifted-cell key
+     * as bits; all k_cell reads) and a 16-byte position record in the
+     * other (x y z, the id as bits).
      * Between a collision and the streaming pass after it the state lives
      * here; after that pass the SoA arrays hold it in the same order, and
      * the records' keys stay the particles' last cell keys. */
-    float *rec;
+    float *rec, *rec1;
 };
 
 #ifdef __HIPCC__
