@@ -211,12 +211,14 @@ def plain_rate(ds, steps, warmup=50):
     return n / t
 
 
-def served(n, rate, steps, budget_gb, plain_every=8, plain_steps=1000):
+def served(n, rate, steps, budget_gb, plain_every=8, plain_steps=1000, options=None):
     """Poisson arrivals of uneven-ensemble requests through the request
     loop, against the plain path on every plain_every-th request (a sample
-    of the same mix, run one after another)."""
+    of the same mix, run one after another); micro-batches of several
+    replicas take the launch-option index `options` (None: the library's
+    defaults)."""
     reqs = serve.poisson_requests(n, rate, steps, seed=1)
-    srv = serve.Server(int(budget_gb * 2 ** 30), chunk=500, max_active=3, window=0.2)
+    srv = serve.Server(int(budget_gb * 2 ** 30), chunk=500, max_active=2, window=0.5, options=options)
     srv.run(reqs)
     out = serve.summary(reqs, srv.makespan)
     sample = [r.replica for r in reqs[::plain_every]]
@@ -225,6 +227,7 @@ def served(n, rate, steps, budget_gb, plain_every=8, plain_steps=1000):
                 "hbm_budget_GiB": budget_gb, "chunk_steps": srv.chunk, "batching_window_s": srv.window,
                 "micro_batches": srv.stats["groups"], "max_concurrent_micro_batches": srv.stats["max_active_groups"],
                 "max_resident_replicas": srv.stats["max_resident_replicas"],
+                "micro_batch_launch_options_index": options,
                 "plain_replica_steps_per_s": plain_rate(sample, plain_steps),
                 "plain_sample": "every %dth request alone (%d of them), %d timed steps each"
                                 % (plain_every, len(sample), plain_steps)})
@@ -239,7 +242,7 @@ def uneven(steps, plain_every=6, plain_steps=1000):
     ds = en.uneven()
     b = en.batch(ds)
     b.set_option(ia.OPT_ALIGN_LISTS, 1)
-    b.set_option(ia.OPT_GRAPHS, 0)
+    kb = b.autotune(20)[0]
     dt = timed(b, steps, 50)
     b.close()
     ns = sum(en.n_solvent(d) for d in ds)
@@ -251,7 +254,7 @@ def uneven(steps, plain_every=6, plain_steps=1000):
             "batched_particle_steps_per_s": ns * steps / dt, "plain_replica_steps_per_s": ra,
             "plain_sample": "every %dth replica alone (%d of them), %d timed steps each"
                             % (plain_every, len(ds[::plain_every]), plain_steps),
-            "speedup": rb / ra}
+            "speedup": rb / ra, "launch_options_index": kb}
 
 
 def trace_summary(path, skip, steps):
@@ -333,7 +336,8 @@ def main():
         print(json.dumps({"uneven": uneven(a.uneven_steps)}))
         return
     if a.only == "served":
-        print(json.dumps({"served": served(a.served_requests, a.served_rate, a.served_steps, 16.0)}))
+        print(json.dumps({"served": served(a.served_requests, a.served_rate, a.served_steps, 16.0,
+                                           options=None if a.options < 0 else a.options)}))
         return
     d = en.single()
     b = ia.Batch([en.spec(d)])
@@ -404,7 +408,10 @@ def main():
     if not a.no_batched:
         line["uneven"] = uneven(a.uneven_steps)
     if not a.no_served:
-        line["served"] = served(a.served_requests, a.served_rate, a.served_steps, 16.0)
+        # micro-batches of several replicas take the launch shape the
+        # autotuner picked for the uneven ensemble, the same mix
+        opt = line["uneven"]["launch_options_index"] if "uneven" in line else None
+        line["served"] = served(a.served_requests, a.served_rate, a.served_steps, 16.0, options=opt)
     print(json.dumps(line))
 
 

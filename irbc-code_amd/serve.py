@@ -62,22 +62,26 @@ class Request:
 class _Group:
     """One context holding the requests admitted together."""
 
-    def __init__(self, reqs, now, graphs=False):
+    def __init__(self, reqs, now, graphs=False, options=None):
         self.reqs = reqs
         self.batch = ia.Batch([r.spec for r in reqs])
-        # a mixed micro-batch's list updates fall on many different steps, so
-        # its chunks take many shapes: captured graphs would rarely be
-        # replayed
-        self.batch.set_option(ia.OPT_GRAPHS, 1 if graphs else 0)
         # list updates (and the near-mask rebuilds after them) beside the
-        # collisions, whatever each replica's cadence
+        # collisions, whatever each replica's cadence: every chunk of a
+        # collision period then has the same shape
         self.batch.set_option(ia.OPT_ALIGN_LISTS, 1)
+        if options is not None and len(reqs) >= 2:
+            # a launch-option index (bit i = irbc_amd.OPTIONS[i]) for
+            # micro-batches of several replicas
+            for i, (_, opt) in enumerate(ia.OPTIONS):
+                self.batch.set_option(opt, (options >> i) & 1)
+        else:
+            self.batch.set_option(ia.OPT_GRAPHS, 1 if graphs else 0)
         for i, r in enumerate(reqs):
             en.load(self.batch, i, r.replica, device_solvent=True)
             if r.series_every > 0:
                 self.batch.set_series(i, r.series_every, max(1, r.steps // r.series_every))
         en.start(self.batch)
-        if len(reqs) >= 8:
+        if len(reqs) >= 8 and options is None:
             # a large micro-batch streams four particles per lane, as the
             # autotuner picks for the 64-replica batch
             self.batch.set_option(ia.OPT_STREAM4, 1)
@@ -89,7 +93,8 @@ class _Group:
 
 
 class Server:
-    def __init__(self, budget_bytes, chunk=250, max_group=64, max_active=2, window=0.0, builders=1, graphs=False):
+    def __init__(self, budget_bytes, chunk=250, max_group=64, max_active=2, window=0.0, builders=1, graphs=False,
+                 options=None):
         self.budget = int(budget_bytes)
         self.chunk = int(chunk)
         self.max_group = int(max_group)
@@ -97,6 +102,7 @@ class Server:
         self.window = float(window)
         self.builders = int(builders)
         self.graphs = bool(graphs)
+        self.options = options
         self.used = 0
         self.active = []
         self.building = []  # (future of a _Group, its reserved bytes)
@@ -134,7 +140,7 @@ class Server:
             # uploads, classification, lists) on a builder thread, while this
             # loop keeps the other micro-batches' chunks going
             self.used += need
-            self.building.append((self.pool.submit(_Group, group, now, self.graphs), need))
+            self.building.append((self.pool.submit(_Group, group, now, self.graphs, self.options), need))
             self.stats["groups"] += 1
 
     def _collect(self, g, now):
