@@ -15,8 +15,8 @@ itself (its trial steps are undone), then the timed run launches without
 timing events; a separate pass with HIP events bound to the kernels gives
 the per-phase figures and the roofline of the dominant kernel (the
 streaming kernel).  Prints ONE JSON line with the throughput, that
-roofline, the CPU oracle's rate on the host cores for a bounded sample of
-the same workload, the batched ensemble (64 replicas of 32x32x24) against
+roofline, the reference's own CPU path on the host cores for a bounded
+sample (with the CPU oracle beside it on the same workload), the batched ensemble (64 replicas of 32x32x24) against
 the same replica run alone in replica-steps/s, and the request loop under
 Poisson arrivals (p50/p99 latency and throughput).
 """
@@ -45,13 +45,59 @@ STREAM_BYTES_PER_PARTICLE = 48.0
 STEP_BYTES_PER_PARTICLE = 48.0 + 24.0 / 10.0
 
 
-def cpu_baseline(b, d, st, budget_s=15.0, max_steps=5000):
-    """The FP64 oracle on the host cores, on the same state, for a bounded sample."""
-    sys.path.insert(0, os.path.join(REPO, "oracle"))
-    import pyoracle as po
+def host_cores():
     ncores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     if os.environ.get("OMP_NUM_THREADS", "").isdigit():
         ncores = min(ncores, int(os.environ["OMP_NUM_THREADS"]))
+    return ncores
+
+
+REF_RUN = os.path.join(REPO, "oracle", "_ref", "ref_run")
+REF_MESH = ("TrophV600.dat", "TrophE600.dat", "TrophT600.dat")
+REF_PARTICLES = 960000  # 60x40x40 cells at density 10 (define.h:43-48)
+
+
+def reference_baseline(ncores, budget_s=12.0):
+    """The reference's own OpenMP CPU path, its sources compiled by
+    oracle/Makefile into oracle/_ref/ref_run, on the host cores: the README
+    run (maincode.cpp:56-770), whose box is fixed at compile time
+    (define.h:43-45), timed over whole steps of its time loop after its
+    set-up, for about budget_s seconds.  None when the build is absent."""
+    import subprocess
+    import tempfile
+    if not os.path.exists(REF_RUN):
+        return None
+    with tempfile.TemporaryDirectory() as tmp:
+        for f in REF_MESH:
+            os.symlink(os.path.join(REPO, "tests", "golden", f), os.path.join(tmp, f))
+        env = dict(os.environ, OMP_NUM_THREADS=str(ncores), IRBC_REF_SEED="12345",
+                   IRBC_REF_TIMING=str(budget_s))
+        t0 = time.perf_counter()
+        r = subprocess.run([REF_RUN, tmp, "100000", "0.005", "0.015", *REF_MESH, "500", "RBC.dat", "0"],
+                           cwd=tmp, env=env, capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise RuntimeError("reference CPU run failed (%d): %s" % (r.returncode, r.stderr[-500:]))
+    tag = [ln.split() for ln in r.stdout.splitlines() if ln.startswith("ref_timing ")]
+    if not tag:
+        raise RuntimeError("reference CPU run printed no timing line")
+    n, sec = int(tag[-1][1]), float(tag[-1][2])
+    return {"value": REF_PARTICLES * n / sec, "unit": "particle-steps/s", "cores": int(ncores),
+            "kind": "reference",
+            "sample": "the reference's OpenMP path (its sources built by oracle/Makefile), README run: "
+                      "60x40x40 cells at density 10 (960000 particles, the box define.h fixes), "
+                      "Troph600 mesh, 500 knobs, kon 0.005 koff 0.015, seed 12345; %d steps in %.1f s "
+                      "after its %.0f s set-up" % (n, sec, wall - sec),
+            "ms_per_step": sec * 1e3 / n}
+
+
+def cpu_baseline(b, d, st, budget_s=10.0, max_steps=5000):
+    """The reference's CPU path on the host cores when its build is present,
+    with the FP64 oracle beside it on the benchmark's own state for a
+    bounded sample; the oracle alone otherwise."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as po
+    ncores = host_cores()
     po.set_threads(ncores)
     p, pp = en.params(d), po.Params()
     for f, _ in ia.Params._fields_:
@@ -78,7 +124,13 @@ def cpu_baseline(b, d, st, budget_s=15.0, max_steps=5000):
                      % (n, n // 10, dt),
            "accuracy": accuracy(b, pp, st, po)}
     out["calibration"] = calibration(out["value"], ncores)
-    return out
+    ref = reference_baseline(ncores)
+    if ref is None:
+        return out
+    ref["port"] = out
+    ref["oracle_over_reference_here"] = ref["ms_per_step"] / 1e3 * out["value"] / REF_PARTICLES
+    ref["accuracy"] = out.pop("accuracy")
+    return ref
 
 
 def calibration(oracle_rate, ncores):

@@ -12,11 +12,16 @@
  *                        n, maincode.cpp:602) — dump the state before it;
  *   force_calculate_dt() same counting — dump the forces after it, then stop
  *                        once the last requested step is written.
+ * With IRBC_REF_TIMING=<seconds> set nothing is dumped: the harness takes the
+ * time at TotalVA() of step 0 and again at each later step, and at step N or
+ * at the first step past that many seconds it prints "ref_timing n seconds"
+ * (n whole steps of the time loop, the set-up excluded) and stops.
  * Dumps are raw little-endian arrays in a simple tagged format read by
  * tests/golden/make_golden.py.
  */
 #include "define.h"
 
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -34,6 +39,9 @@ static int g_totalva_calls = 0;
 static int g_force_calls = 0;
 static long g_dump_step = -1;
 static std::string g_dump_dir;
+static bool g_timing = false;
+static double g_timing_budget = 0.0;
+static std::chrono::steady_clock::time_point g_t0;
 
 extern "C" time_t __wrap_time(time_t *t)
 {
@@ -126,7 +134,18 @@ extern "C" void __wrap__Z7TotalVAv()
 {
     ++g_totalva_calls;
     long step = (long)g_totalva_calls - 2; /* call 1 is the init call */
-    if (step >= 0 && step == g_dump_step) dump_state("pre_va", step);
+    if (g_timing) {
+        const auto now = std::chrono::steady_clock::now();
+        if (step == 0) g_t0 = now;
+        const double el = std::chrono::duration<double>(now - g_t0).count();
+        if (step > 0 && (step == g_dump_step || el >= g_timing_budget)) {
+            std::printf("ref_timing %ld %.9f\n", step, el);
+            std::fflush(stdout);
+            std::_Exit(0);
+        }
+    } else if (step >= 0 && step == g_dump_step) {
+        dump_state("pre_va", step);
+    }
     __real__Z7TotalVAv();
 }
 
@@ -135,6 +154,7 @@ extern "C" void __wrap__Z18force_calculate_dtv()
     __real__Z18force_calculate_dtv();
     ++g_force_calls;
     long step = (long)g_force_calls - 2;
+    if (g_timing) return;
     if (step == -1) dump_state("init_forces", 0);
     if (step >= 0 && step == g_dump_step) {
         dump_state("post_forces", step);
@@ -152,6 +172,10 @@ int main(int argc, char **argv)
     }
     g_dump_dir = argv[1];
     g_dump_step = std::atol(argv[2]);
+    if (const char *tb = std::getenv("IRBC_REF_TIMING")) {
+        g_timing = true;
+        g_timing_budget = std::atof(tb);
+    }
     std::vector<char *> args;
     args.push_back(argv[0]);
     for (int i = 3; i < argc; ++i) args.push_back(argv[i]);
