@@ -112,10 +112,11 @@ int irbc_get_parasite(irbc_ctx *ctx, int r, double *d, int32_t *i);
 int64_t irbc_n_ligands(const irbc_ctx *ctx, int r);
 int irbc_set_ligands(irbc_ctx *ctx, int r, const int32_t *ligi);
 int irbc_get_ligands(irbc_ctx *ctx, int r, int32_t *ligi);
-/* g[12] = Sarea, Volume, Sarea0, Volume0, theta0, wallspeed, status,
+/* g[13] = Sarea, Volume, Sarea0, Volume0, theta0, wallspeed, status,
  * bond draws of the last step, step, bonded knobs, particles queued for
  * bounce-back in the last step, the XCD the replica's last membrane chain
- * ran on (-1: none, or device-scope hand-offs) */
+ * ran on (-1: none, or device-scope hand-offs), how the last collision ran
+ * (1: sorted, or none yet; 2: in place, see IRBC_OPT_SORT_EVERY) */
 int irbc_get_globals(irbc_ctx *ctx, int r, double *g);
 /* the time-loop counters before the top-of-step increment (maincode.cpp:450-475) */
 int irbc_set_counters(irbc_ctx *ctx, int r, int64_t step, int32_t colsteps, int32_t upd);
@@ -236,6 +237,16 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
  *   membrane moves less than the lists' margin between updates, the
  *   reference's own assumption. */
 #define IRBC_OPT_ALIGN_LISTS 13
+/* IRBC_OPT_SORT_EVERY  value k >= 1 (default 1): the collision of a step
+ *   sorts the particles into cell order when (step / csteps) is a multiple
+ *   of k, i.e. every k-th collision; the others leave each particle in its
+ *   slot and reduce the cell sums with LDS-aggregated 64-bit atomics.  The
+ *   cell sums are fixed point, so the results are the same for every k.
+ *   On an MI355X the in-place collision is the slower one (about 160 us
+ *   against 111 us on a 64x64x48 box: its four atomics per cell and tile
+ *   outweigh the sort's data movement), hence the default.
+ *   Stage-by-stage collisions (irbc_collision_step) always sort. */
+#define IRBC_OPT_SORT_EVERY 14
 int irbc_set_option(irbc_ctx *ctx, int option, int value);
 /* Times `steps` MD steps under 48 combinations of the options above on
  * this context's own replicas, then keeps the fastest: the membrane as the

@@ -184,7 +184,7 @@ int64_t bytes_of(const Sizes &z, int n)
 {
     int64_t b = 0;
     b += z.slots * (6 * 4 + 4 + 32 + 3 * 4);              /* particles, ids, sort records, keys, ranks, queue */
-    b += z.cells * (4 + 8 * 8 + 4 + 4 + 8 * 8 + 4 + 8 + 4); /* count, ctab, vhead, llcount, diag, cursor, near mask, listed */
+    b += z.cells * (4 + 8 * 8 + 4 + 4 + 8 * 8 + 4 + 8 + 4 + 32); /* count, ctab, vhead, llcount, diag, cursor, near mask, listed, sums */
     b += z.cells * 8 * 8;                                 /* tapes */
     b += z.cells1 * (4 + 4);                              /* start, lloff */
     b += z.llent * 4;
@@ -219,6 +219,7 @@ struct irbc_ctx {
     int one_queue = 1;          /* with the chain: the step on one queue, the membrane inside the solvent's launches */
     int classify_all = 0;       /* classification tests every triangle (1) or the binned ones (0) */
     int align_lists = 0;        /* list updates at collision steps (lists_due) */
+    int sort_every = IRBC_SORT_EVERY_DEFAULT; /* collisions per counting sort (sorts_at) */
     int ncu = 256;              /* compute units of the device */
     /* the device state is as the last irbc_md_steps left it (vertex and
      * triangle kinematics, near masks, drift margins, the next step's grid
@@ -244,6 +245,7 @@ struct irbc_ctx {
     uint32_t *ida;
     float *rec; /* the collision's sorted records, 8 floats per slot */
     int32_t *skey, *srank, *nearq, *deferq;
+    unsigned long long *csum; /* per cell: fixed-point momentum x y z and dvv of an in-place collision */
     /* cells */
     int32_t *ccount, *cstart, *vhead, *lloff, *llcount, *llent;
     double *ctab;
@@ -386,6 +388,7 @@ static void top_of_step(const irbc_params &p, int64_t step, int32_t colsteps_pre
     R.upd = upd_pre + 1;
     R.collide = R.colsteps == p.csteps;
     if (R.collide) R.colsteps = 0;
+    R.sort_now = R.collide && sorts_at(step, p.csteps, R.sort_every);
     R.llist = lists_due(R.upd, R.collide != 0, p.csteps, p.uplist, R.align_lists);
     if (R.llist) R.upd = 0;
 }
@@ -493,7 +496,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     A(c->rec, 8 * z.slots)
     A(c->ida, z.slots) A(c->skey, z.slots) A(c->srank, z.slots) A(c->nearq, z.slots)
     A(c->deferq, 2 * z.slots)
-    A(c->ccount, z.cells) A(c->cstart, z.cells1) A(c->vhead, z.cells) A(c->lloff, z.cells1)
+    A(c->ccount, z.cells) A(c->csum, 4 * z.cells) A(c->cstart, z.cells1) A(c->vhead, z.cells) A(c->lloff, z.cells1)
     A(c->llcount, z.cells) A(c->llent, z.llent) A(c->ctab, z.cells * 8) A(c->cmask, z.cells) A(c->listed, z.cells)
     A(c->diag.vcm, 3 * z.cells) A(c->diag.ncc, z.cells) A(c->diag.rand1, z.cells)
     A(c->diag.rand2, z.cells) A(c->diag.dvv, z.cells) A(c->diag.the, z.cells) A(c->diag.nc, z.cells)
@@ -550,6 +553,9 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
         R.ida = c->ida;
         R.rec = c->rec;
         R.rec1 = c->rec + 4 * c->z.slots;
+        R.skeya = c->skey;
+        R.sort_every = c->sort_every;
+        R.key_src = 1;
         top_of_step(p, 0, -1, 0, R);
         /* ligand grid coordinates (maincode.cpp:77-93) */
         for (double m = 0.0; m < p.Lx; m += p.lig_spacing) lx.push_back(m);
@@ -915,6 +921,7 @@ int irbc_set_solvent(irbc_ctx *c, int r, int64_t n, const double *x, const doubl
     HIPCHK(hipMemset(R.rec + 4 * R.poff, 0, (size_t)n * 16));
     HIPCHK(hipMemset(R.rec1 + 4 * R.poff, 0, (size_t)n * 16));
     R.rotate_pending = 0;
+    R.key_src = 1;
     c->hr[r].solvent = true;
     return c->push(r);
 }
@@ -939,6 +946,7 @@ int irbc_init_solvent(irbc_ctx *c, int r, int maxwell)
     rc = c->pull(r); /* the scan epoch the binning advanced */
     if (rc) return rc;
     c->rep[r].rotate_pending = 0;
+    c->rep[r].key_src = 1;
     c->hr[r].solvent = true;
     return c->push(r);
 }
@@ -1050,6 +1058,7 @@ int irbc_get_globals(irbc_ctx *c, int r, double *g)
     g[9] = R.nbonds;
     g[10] = R.near_count[c->par ^ 1];
     g[11] = R.chain_xcc_last;
+    g[12] = R.key_src;
     return IRBC_OK;
 }
 
@@ -1197,8 +1206,13 @@ static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t 
 /* grid for a kernel that maps blocks through xcd_tile */
 static unsigned xcd_grid(unsigned n) { return IRBC_NXCD * ((n + IRBC_NXCD - 1) / IRBC_NXCD); }
 
-static int launch_collision(irbc_ctx *c, int staged, int diag)
+/* Collision flags of a step: which kinds of collision its replicas run. */
+#define COL_SORT 1    /* a replica's collision sorts (and every staged one) */
+#define COL_INPLACE 2 /* a replica's collision leaves its particles in place */
+
+static int launch_collision(irbc_ctx *c, int staged, int diag, int cf)
 {
+    if (staged) cf = COL_SORT;
     const unsigned T = (unsigned)c->z.tiles, CT = (unsigned)c->z.ctiles;
     if (staged) k_draw_shift<<<(unsigned)((c->nrep + 63) / 64), 64, 0, c->st>>>(c->d_rep, c->nrep, 0);
     const int xc = c->xcd_chunk;
@@ -1208,17 +1222,25 @@ static int launch_collision(irbc_ctx *c, int staged, int diag)
      * their count is a multiple of IRBC_NXCD */
     const unsigned ndb = xcd_grid(CT);
     k_bin<<<nvb + ndb + (xc ? xcd_grid(T) : T), IRBC_WG, 0, c->st>>>(
-        c->d_rep, c->nrep, (int)nvb, c->d_tile_rep, (int)T, xc, staged, c->skey, c->srank, c->ccount, M, c->vhead,
-        c->vnext, c->d_ctile_rep, (int)CT, (int)ndb, c->cell_u, c->ctab, c->wall_z, diag, c->diag);
-    int rc = launch_scan(c, staged, 0, c->ccount, c->cstart);
-    if (rc) return rc;
-    k_scatter<<<xc ? xcd_grid(T) : T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, xc, staged, c->skey,
-                                                           c->srank, c->cstart, -1);
+        c->d_rep, c->nrep, (int)nvb, c->d_tile_rep, (int)T, xc, staged, c->skey, c->srank, c->ccount, c->csum, M,
+        c->vhead, c->vnext, c->d_ctile_rep, (int)CT, (int)ndb, c->cell_u, c->ctab, c->wall_z, diag, c->diag);
+    if (cf & COL_SORT) {
+        int rc = launch_scan(c, staged, 0, c->ccount, c->cstart);
+        if (rc) return rc;
+        k_scatter<<<xc ? xcd_grid(T) : T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, xc, staged, c->skey,
+                                                               c->srank, c->cstart, -1);
+    }
     k_cell<<<xc ? xcd_grid(CT) : CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, xc, staged, c->cstart,
-                                                         c->vhead,
-                                                c->vnext,
-                                      c->mvx, c->mvy, c->mvz, c->wall_z, c->mem_z, c->ctab,
-                                      diag, c->diag, c->par);
+                                                         c->vhead, c->vnext, c->mvx, c->mvy, c->mvz, c->wall_z,
+                                                         c->mem_z, c->ctab, diag, c->diag, c->par, c->ccount,
+                                                         c->csum);
+    if (cf & COL_INPLACE) {
+        k_dvv<<<xc ? xcd_grid(T) : T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, xc, c->skey, c->ctab,
+                                                           c->csum);
+        k_cell_the<<<xc ? xcd_grid(CT) : CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, xc, c->vhead,
+                                                                 c->vnext, c->mvx, c->mvy, c->mvz, c->ctab, diag,
+                                                                 c->diag, c->ccount, c->csum);
+    }
     k_vrot<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, staged, M, c->ctab);
     if (staged) k_rotate<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->ctab);
     LAUNCHCHK();
@@ -1454,7 +1476,7 @@ int irbc_collision_step(irbc_ctx *c)
     c->primed = false;
     int rc = ready(c, true);
     if (rc) return rc;
-    rc = launch_collision(c, 1, 1);
+    rc = launch_collision(c, 1, 1, COL_SORT);
     if (rc) return rc;
     return irbc_sync(c);
 }
@@ -2023,6 +2045,7 @@ int irbc_checkpoint_load(irbc_ctx *c, int r, const void *buf, int64_t n)
     HIPCHK(hipMemset(R.rec + 4 * R.poff, 0, (size_t)H.ns * 16));
     HIPCHK(hipMemset(R.rec1 + 4 * R.poff, 0, (size_t)H.ns * 16));
     R.rotate_pending = 0;
+    R.key_src = 1;
     R.seed = H.seed;
     R.tapes = H.tapes;
     R.status = 0;
@@ -2058,9 +2081,10 @@ int irbc_checkpoint_load(irbc_ctx *c, int r, const void *buf, int64_t n)
  * means.  `first` skips the wait for the previous step's membrane (the
  * caller has joined the chains), `rec` brackets phases with timing events.
  * join_chains() makes the solvent stream wait for the last membrane. */
-static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec, bool first)
+static int launch_step(irbc_ctx *c, int cf, bool ll, bool rec, bool first)
 {
     int rc;
+    const bool col = cf != 0;
     const int p = c->par & 1;
     const int mem_phases = PH_ALL & ~PH_RESET_NEAR; /* k_stream clears the other parity's queue */
     const bool lock = col || ll || first;
@@ -2068,7 +2092,7 @@ static int launch_step(irbc_ctx *c, bool col, bool ll, bool rec, bool first)
     if (!first) HIPCHK(hipStreamWaitEvent(c->st, c->ev_mem2[p ^ 1], 0));
     if (col) {
         if (rec) { e0 = c->event(); hipEventRecord(e0, c->st); }
-        rc = launch_collision(c, 0, c->diag_on ? 1 : 0);
+        rc = launch_collision(c, 0, c->diag_on ? 1 : 0, cf);
         if (rc) return rc;
         if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[1].push_back({e0, e1}); }
     }
@@ -2255,14 +2279,15 @@ static int launch_step_near(irbc_ctx *c, int phases)
  * near masks after either], the fused streaming launch, the fused
  * bounce-back launch.  rec brackets phases with timing events (the
  * membrane runs inside the streaming launch and has no figure of its own). */
-static int launch_step_one(irbc_ctx *c, bool col, bool ll, bool rec)
+static int launch_step_one(irbc_ctx *c, int cf, bool ll, bool rec)
 {
     int rc;
+    const bool col = cf != 0;
     const int mem_phases = PH_ALL & ~PH_RESET_NEAR; /* k_stream clears the other parity's queue */
     hipEvent_t e0 = nullptr, e1 = nullptr, s0 = nullptr;
     if (col) {
         if (rec) { e0 = c->event(); hipEventRecord(e0, c->st); }
-        rc = launch_collision(c, 0, c->diag_on ? 1 : 0);
+        rc = launch_collision(c, 0, c->diag_on ? 1 : 0, cf);
         if (rc) return rc;
         if (rec) { e1 = c->event(); hipEventRecord(e1, c->st); c->pend[1].push_back({e0, e1}); }
     }
@@ -2306,11 +2331,13 @@ static int join_chains(irbc_ctx *c)
 
 /* collide / list-update flags of the next step of every replica, then
  * the host mirror advanced by one step */
-static void next_flags(irbc_ctx *c, std::vector<HostRep> &hr, bool &col, bool &ll)
+static void next_flags(irbc_ctx *c, std::vector<HostRep> &hr, int &cf, bool &ll)
 {
-    col = ll = false;
+    cf = 0;
+    ll = false;
     for (int r = 0; r < c->nrep; ++r) {
-        col |= host_collides(hr[r].spec.params, hr[r]);
+        const irbc_params &p = hr[r].spec.params;
+        if (host_collides(p, hr[r])) cf |= sorts_at(hr[r].step, p.csteps, c->sort_every) ? COL_SORT : COL_INPLACE;
         ll |= host_llist(hr[r].spec.params, hr[r], c->align_lists);
         advance_host(hr[r].spec.params, hr[r], c->align_lists);
     }
@@ -2321,7 +2348,7 @@ static void next_flags(irbc_ctx *c, std::vector<HostRep> &hr, bool &col, bool &l
  * each step, the starting parity and the launch options. */
 static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
 {
-    /* first: 2 flag bits per step; second: parity, chunk length and every
+    /* first: 4 flag bits per step; second: parity, chunk length and every
      * option that changes the launches, each in a field of its own */
     const uint64_t shape = (uint64_t)(c->par & 1) | ((uint64_t)n << 1) | ((uint64_t)(c->mem_fused & 1) << 8) |
                            ((uint64_t)(c->diag_on & 1) << 9) | ((uint64_t)(c->near_merged & 1) << 10) |
@@ -2336,8 +2363,9 @@ static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
         HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
         int rc = IRBC_OK;
         for (int k = 0; k < n && rc == IRBC_OK; ++k) {
-            const bool col = (flags >> (2 * k)) & 1, ll = (flags >> (2 * k + 1)) & 1;
-            rc = one_queue(c) ? launch_step_one(c, col, ll, false) : launch_step(c, col, ll, false, k == 0);
+            const int cf = (int)((flags >> (4 * k)) & 3);
+            const bool ll = (flags >> (4 * k + 2)) & 1;
+            rc = one_queue(c) ? launch_step_one(c, cf, ll, false) : launch_step(c, cf, ll, false, k == 0);
             c->par ^= 1;
         }
         if (rc == IRBC_OK) rc = join_chains(c); /* the capture ends with both streams joined */
@@ -2392,9 +2420,10 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
             const int n = (int)std::min<int64_t>(CH, nsteps - s);
             uint64_t flags = 0;
             for (int k = 0; k < n; ++k) {
-                bool col, ll;
-                next_flags(c, c->hr, col, ll);
-                flags |= (uint64_t)col << (2 * k) | (uint64_t)ll << (2 * k + 1);
+                int cf;
+                bool ll;
+                next_flags(c, c->hr, cf, ll);
+                flags |= (uint64_t)cf << (4 * k) | (uint64_t)ll << (4 * k + 2);
             }
             rc = run_graph_chunk(c, n, flags);
             if (rc) return rc;
@@ -2404,10 +2433,11 @@ int irbc_md_steps(irbc_ctx *c, int64_t nsteps)
         return IRBC_OK;
     }
     for (int64_t s = 0; s < nsteps; ++s) {
-        bool col, ll;
-        next_flags(c, c->hr, col, ll);
+        int cf;
+        bool ll;
+        next_flags(c, c->hr, cf, ll);
         const bool rec = c->prof && (c->prof_step++ % c->prof_every) == 0;
-        rc = one_queue(c) ? launch_step_one(c, col, ll, rec) : launch_step(c, col, ll, rec, s == 0);
+        rc = one_queue(c) ? launch_step_one(c, cf, ll, rec) : launch_step(c, cf, ll, rec, s == 0);
         if (rc) return rc;
         c->par ^= 1;
         if (c->prof && c->pend[0].size() >= 2048) {
@@ -2473,6 +2503,20 @@ int irbc_set_option(irbc_ctx *c, int option, int value)
     case IRBC_OPT_CLASSIFY_ALL:
         c->classify_all = value != 0;
         return IRBC_OK;
+    case IRBC_OPT_SORT_EVERY: {
+        if (value < 1) return fail(IRBC_ERR_ARG, "irbc_set_option: IRBC_OPT_SORT_EVERY needs a value >= 1");
+        HIPCHK(c->sync_both());
+        HIPCHK(hipMemcpy(c->rep.data(), c->d_rep, c->nrep * sizeof(DevRep), hipMemcpyDeviceToHost));
+        c->sort_every = value;
+        for (int r = 0; r < c->nrep; ++r) {
+            DevRep &R = c->rep[r];
+            const HostRep &h = c->hr[r];
+            R.sort_every = value;
+            top_of_step(h.spec.params, h.step, h.colsteps, h.upd, R);
+        }
+        HIPCHK(hipMemcpy(c->d_rep, c->rep.data(), c->nrep * sizeof(DevRep), hipMemcpyHostToDevice));
+        return IRBC_OK;
+    }
     case IRBC_OPT_ALIGN_LISTS: {
         /* the device's copy of the rule for every replica, and the step
          * about to run re-decided under it, as the host mirror decides */

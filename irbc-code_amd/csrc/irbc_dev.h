@@ -26,6 +26,9 @@
 #define IRBC_RANDNO 12345678   /* unbound-ligand sentinel (define.h:37) */
 #define IRBC_FIX_SCALE 1099511627776.0      /* 2^40: fixed-point unit of cell sums */
 #define IRBC_FIX_INV (1.0 / 1099511627776.0)
+#ifndef IRBC_SORT_EVERY_DEFAULT
+#define IRBC_SORT_EVERY_DEFAULT 1 /* collisions per counting sort (IRBC_OPT_SORT_EVERY) */
+#endif
 
 /* status bits raised by device code, read by the host after a sync */
 #define IRBC_ST_NAN 1
@@ -96,9 +99,12 @@ struct DevRep {
     int64_t step;           /* index of the step about to run */
     int32_t colsteps, upd;  /* counters after this step's increment */
     int32_t collide, llist; /* this step runs a collision / a list update */
-    int32_t rotate_pending; /* collision done: the sorted state is in the records (rec), rotated and
-                             * written back by the next streaming pass */
-    int32_t rec_pad;
+    int32_t rotate_pending; /* collision done, its rotation applied by the next streaming pass: 1 the
+                             * sorted state is in the records (rec), written back in their order;
+                             * 2 the particles stayed in their slots, keys in skeya */
+    int32_t sort_now;       /* this step's collision sorts the particles into cell order (sorts_at) */
+    int32_t sort_every;     /* collisions per sort (1: every collision sorts) */
+    int32_t key_src;        /* where the last collision's keys are: 1 the records, 2 skeya */
     double wall2[2];        /* top-wall speed of this and the next step, by step parity (the
                              * membrane part of step s writes that of step s+1) */
     double shift[3];
@@ -148,6 +154,10 @@ struct DevRep {
      * here; after that pass the SoA arrays hold it in the same order, and
      * the records' keys stay the particles' last cell keys. */
     float *rec, *rec1;
+    /* A collision that does not sort leaves the particles in their slots
+     * and writes each one's shifted-cell key here, slot by slot. */
+    int32_t *skeya;
+    int32_t skey_pad[2];
 };
 
 #ifdef __HIPCC__
@@ -166,6 +176,18 @@ struct DevRep {
 __host__ __device__ inline bool lists_due(int32_t upd_after, bool collide, int32_t csteps, int32_t uplist, int align)
 {
     return upd_after == uplist || (align && collide && upd_after + csteps > uplist);
+}
+
+/* Does the collision of step `step` (a collision step) sort the particles
+ * into cell order?  Every sort_every-th collision does: consecutive
+ * collisions lie csteps apart, so step / csteps advances by one between
+ * them.  The results do not depend on it (the cell sums are fixed point and
+ * every particle's arithmetic is its own); the sort keeps a tile's particles
+ * in few cells, which the collision's LDS aggregation and the streaming
+ * pass's cell-table reads rely on for their speed. */
+__host__ __device__ inline bool sorts_at(int64_t step, int32_t csteps, int32_t every)
+{
+    return every <= 1 || (step / (csteps > 0 ? csteps : 1)) % every == 0;
 }
 
 /* ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) ---- */

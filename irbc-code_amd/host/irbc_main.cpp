@@ -141,7 +141,7 @@ int main(int argc, char **argv)
         }
     }
     const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    double g[12];
+    double g[13];
     irbc_get_globals(c, 0, g);
     std::fprintf(stderr, "irbc_main: %lld steps in %.3f s (%.1f us/step, %.3g particle-steps/s); "
                          "area %.6f volume %.6f bonds %d\n",

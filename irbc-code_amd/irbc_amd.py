@@ -44,6 +44,7 @@ OPT_NEAR_MASKS = 10
 OPT_ONE_QUEUE = 11
 OPT_CLASSIFY_ALL = 12
 OPT_ALIGN_LISTS = 13
+OPT_SORT_EVERY = 14
 OPTIONS = (("membrane_fused", 1), ("near_merged", 3), ("graphs", 2), ("xcd_chunk", 4), ("stream4", 5),
            ("stream_grid", 6), ("mem_after_stream", 7), ("membrane_chain", 8), ("one_queue", 11))
 
@@ -317,10 +318,10 @@ class Batch:
         _chk(self.L.irbc_set_ligands(self.h, r, _ptr(a, _i)))
 
     def globals(self, r=0):
-        g = np.zeros(12)
+        g = np.zeros(13)
         _chk(self.L.irbc_get_globals(self.h, r, _ptr(g, _d)))
         return dict(zip(("Sarea", "Volume", "Sarea0", "Volume0", "theta0", "wallspeed", "status",
-                         "bond_draws", "step", "nbonds", "near_queue", "chain_xcd"), g))
+                         "bond_draws", "step", "nbonds", "near_queue", "chain_xcd", "collision_kind"), g))
 
     def set_counters(self, r, step, colsteps, upd):
         _chk(self.L.irbc_set_counters(self.h, r, int(step), int(colsteps), int(upd)))

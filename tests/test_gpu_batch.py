@@ -195,6 +195,55 @@ def test_near_masks_never_change_results():
 
 
 @pytest.mark.gpu
+def test_sort_schedule_never_changes_results():
+    """Collisions that leave the particles in their slots (k_bin's LDS-summed
+    momentum, k_dvv, k_cell_the, the rotation in place) give the same bits
+    as collisions that sort every time: 120 steps (12 collisions, list
+    updates) of an uneven batch with the collision diagnostics on, sorting
+    every collision, every third (in place in between, in both streaming
+    shapes, on one queue with graphs and on two queues), and never after the
+    first.  Solvent state, keys, vertices, bonds, counters and the last
+    collision's cell parameters all equal."""
+    ds = _mix() + [en.replica(20, 20, 16, 10, 642, None, 1.0, 300, kon=5.0, koff=1.0, shear=0.25, seed=31)]
+    runs = []
+    for every, opts in ((1, {}), (3, {}), (3, {ia.OPT_STREAM4: 1}),
+                        (3, {ia.OPT_ONE_QUEUE: 0, ia.OPT_GRAPHS: 0}), (1000, {ia.OPT_STREAM4: 1})):
+        print("sort every", every, opts, flush=True)
+        b = en.batch(ds)
+        b.set_option(ia.OPT_SORT_EVERY, every)
+        for k, v in opts.items():
+            b.set_option(k, v)
+        b.set_diagnostics(True)
+        kinds = []
+        for _ in range(12):
+            b.md_steps(10)
+            kinds.append(b.globals(0)["collision_kind"])
+        # call k runs steps 10k .. 10k + 9, the collision (csteps 10) at
+        # step 10k first; it sorts when k is a multiple of the schedule (none
+        # before the first collision reads as sorted)
+        assert kinds == [1 if k % every == 0 else 2 for k in range(12)], kinds
+        snap = [_snapshot(b, r) for r in range(len(ds))]
+        for r in range(len(ds)):
+            snap[r].update({"cell_" + k: a for k, a in b.cells(r).items()})
+        runs.append(snap)
+        b.close()
+    for run in runs[1:]:
+        for r in range(len(ds)):
+            for key in runs[0][r]:
+                np.testing.assert_array_equal(run[r][key], runs[0][r][key], err_msg="replica %d %s" % (r, key))
+
+
+@pytest.mark.gpu
+def test_sort_every_rejects_zero():
+    b = en.batch(_mix()[:1])
+    try:
+        with pytest.raises(ia.IrbcError, match="SORT_EVERY"):
+            b.set_option(ia.OPT_SORT_EVERY, 0)
+    finally:
+        b.close()
+
+
+@pytest.mark.gpu
 def test_autotune_leaves_the_trajectory_alone():
     """The autotuner's trial steps are undone: a batch tuned and then run
     equals a batch run with the chosen options from the start."""
