@@ -2177,6 +2177,9 @@ static bool one_queue(const irbc_ctx *c) { return c->one_queue && c->mem_chain; 
 #ifndef IRBC_CHAIN_RESIDENCY
 #define IRBC_CHAIN_RESIDENCY 1.0 /* share of the launch a chain workgroup holds its wave slots */
 #endif
+#ifndef IRBC_CHAIN_RESIDENCY_SOLO
+#define IRBC_CHAIN_RESIDENCY_SOLO 5.3 /* the same for a single replica: its XCD keeps ~9% of a share */
+#endif
 /* The bulk units' split over the XCDs for the fused streaming launch, a
  * contiguous range per XCD.  With fewer replicas than XCDs some XCDs host
  * no chain, and XCD x's weight is 1 - IRBC_CHAIN_RESIDENCY x (the share of
@@ -2185,10 +2188,18 @@ static bool one_queue(const irbc_ctx *c) { return c->one_queue && c->mem_chain; 
  * equal.  Measured (us per step, one box): the 64x64x48 replica 59.3 with
  * the tiles dealt round-robin, 58.8 with equal ranges, 56.8 weighted; a
  * mixed 16-request group 293 round-robin, 257 equal, 307 weighted; the
- * 64-replica batch unchanged. */
+ * 64-replica batch unchanged.
+ * A single replica's chain is a string of dependent round trips through its
+ * XCD's L2, each slower while that XCD also streams: leaving it little of
+ * the bulk (IRBC_CHAIN_RESIDENCY_SOLO) shortens the chain more than the
+ * other seven XCDs' larger shares lengthen the sweep.  On the 64x64x48
+ * replica (one box, us per step, one and four particles per lane): 54.0 and
+ * 54.0 with the slot-share weight, 51.2 and 51.6 at 4.0, 50.5 and 51.0 at
+ * 5.3, 51.0 and 50.3 with no bulk at all on that XCD. */
 static XcdSplit xcd_split(const irbc_ctx *c, int units)
 {
-    const double res = c->nrep < IRBC_NXCD ? IRBC_CHAIN_RESIDENCY : 0.0;
+    const double res = c->nrep == 1 ? IRBC_CHAIN_RESIDENCY_SOLO
+                                    : (c->nrep < IRBC_NXCD ? IRBC_CHAIN_RESIDENCY : 0.0);
     const double slots = 32.0 * 4.0 * 4.0 / 4.0; /* 32 CUs x 4 SIMDs x 4 waves, in 4-wave workgroups */
     double w[IRBC_NXCD], ws = 0.0;
     for (int x = 0; x < IRBC_NXCD; ++x) w[x] = 0.0;
@@ -2200,7 +2211,7 @@ static XcdSplit xcd_split(const irbc_ctx *c, int units)
         w[r % IRBC_NXCD] += vb + (eb + tb + vb) + (R.nv + VPB - 1) / VPB; /* the chain's workgroups */
     }
     for (int x = 0; x < IRBC_NXCD; ++x) {
-        w[x] = 1.0 - res * std::min(0.9, w[x] / slots);
+        w[x] = std::max(0.05, 1.0 - res * std::min(0.9, w[x] / slots));
         ws += w[x];
     }
     XcdSplit xs;
