@@ -263,27 +263,36 @@ def plain_rate(ds, steps, warmup=50):
     return n / t
 
 
-def served(n, rate, steps, budget_gb, plain_every=8, plain_steps=1000, options=None):
+def served(n, rate, steps, budget_gb, plain_every=8, plain_steps=1000, options=None, repack=True, chunk=250,
+           window=0.1):
     """Poisson arrivals of uneven-ensemble requests through the request
     loop, against the plain path on every plain_every-th request (a sample
-    of the same mix, run one after another); micro-batches of several
-    replicas take the launch-option index `options` (None: the library's
-    defaults)."""
+    of the same mix, run one after another); contexts of several replicas
+    take the launch-option index `options` (None: the library's defaults).
+    repack: every running request in one context, re-packed at admissions
+    and finishes; else micro-batches side by side."""
     reqs = serve.poisson_requests(n, rate, steps, seed=1)
-    srv = serve.Server(int(budget_gb * 2 ** 30), chunk=500, max_active=2, window=0.5, options=options)
+    srv = serve.Server(int(budget_gb * 2 ** 30), chunk=chunk, max_active=2, window=window, options=options,
+                       repack=repack)
     srv.run(reqs)
     out = serve.summary(reqs, srv.makespan)
     sample = [r.replica for r in reqs[::plain_every]]
     out.update({"workload": "%d requests, Poisson arrivals at %.1f/s, %d steps each, boxes 24^3..64x64x48 "
                             "at density 5-10, 162/642/2562-vertex ellipsoids" % (n, rate, steps),
+                "schedule": "re-packing: one running context, rebuilt at admissions and finishes" if repack
+                else "micro-batches side by side",
                 "hbm_budget_GiB": budget_gb, "chunk_steps": srv.chunk, "batching_window_s": srv.window,
-                "micro_batches": srv.stats["groups"], "max_concurrent_micro_batches": srv.stats["max_active_groups"],
+                "contexts_built": srv.stats["groups"], "max_concurrent_contexts": srv.stats["max_active_groups"],
                 "max_resident_replicas": srv.stats["max_resident_replicas"],
-                "micro_batch_launch_options_index": options,
+                "launch_options_index": options,
                 "plain_replica_steps_per_s": plain_rate(sample, plain_steps),
                 "plain_sample": "every %dth request alone (%d of them), %d timed steps each"
                                 % (plain_every, len(sample), plain_steps)})
     out["speedup_vs_plain"] = out["replica_steps_per_s"] / out["plain_replica_steps_per_s"]
+    if repack:
+        out["idle_waiting_for_build_s"] = srv.idle
+        out["contexts"] = srv.log
+        out["chunks"] = srv.chunks
     return out
 
 
@@ -365,6 +374,11 @@ def main():
     ap.add_argument("--served-requests", type=int, default=64)
     ap.add_argument("--served-rate", type=float, default=32.0)
     ap.add_argument("--served-steps", type=int, default=5000)
+    ap.add_argument("--served-chunk", type=int, default=250)
+    ap.add_argument("--served-window", type=float, default=0.1)
+    ap.add_argument("--served-budget-gb", type=float, default=64.0)
+    ap.add_argument("--served-microbatches", action="store_true",
+                    help="serve with micro-batches side by side instead of re-packing")
     ap.add_argument("--profile-every", type=int, default=4)
     ap.add_argument("--options", type=int, default=-1,
                     help="launch-option index (bit i = irbc_amd.OPTIONS[i]) instead of autotuning")
@@ -388,8 +402,10 @@ def main():
         print(json.dumps({"uneven": uneven(a.uneven_steps)}))
         return
     if a.only == "served":
-        print(json.dumps({"served": served(a.served_requests, a.served_rate, a.served_steps, 16.0,
-                                           options=None if a.options < 0 else a.options)}))
+        print(json.dumps({"served": served(a.served_requests, a.served_rate, a.served_steps, a.served_budget_gb,
+                                           options=None if a.options < 0 else a.options,
+                                           repack=not a.served_microbatches, chunk=a.served_chunk,
+                                           window=a.served_window)}))
         return
     d = en.single()
     b = ia.Batch([en.spec(d)])
@@ -463,7 +479,8 @@ def main():
         # micro-batches of several replicas take the launch shape the
         # autotuner picked for the uneven ensemble, the same mix
         opt = line["uneven"]["launch_options_index"] if "uneven" in line else None
-        line["served"] = served(a.served_requests, a.served_rate, a.served_steps, 16.0, options=opt)
+        line["served"] = served(a.served_requests, a.served_rate, a.served_steps, a.served_budget_gb, options=opt,
+                                repack=not a.served_microbatches, chunk=a.served_chunk, window=a.served_window)
     print(json.dumps(line))
 
 

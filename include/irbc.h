@@ -275,6 +275,14 @@ int64_t irbc_checkpoint_bytes(irbc_ctx *ctx, int r);
 int64_t irbc_checkpoint_save(irbc_ctx *ctx, int r, void *buf, int64_t cap);
 /* the replica's mesh must be loaded (irbc_set_mesh) first */
 int irbc_checkpoint_load(irbc_ctx *ctx, int r, const void *buf, int64_t n);
+/* A replica moved device to device: replica rs of src into slot rd of dst
+ * (same device, same sizes and parameters): the state a checkpoint holds,
+ * the mesh (dst needs no irbc_set_mesh) and the time series (cadence,
+ * capacity, records).  src is left as it was.  dst then continues bit for
+ * bit as src would have.  A request loop re-packs its running replicas
+ * with it into a new context when requests arrive or finish (continuous
+ * batching in one launch sequence). */
+int irbc_copy_replica(irbc_ctx *dst, int rd, irbc_ctx *src, int rs);
 
 /* ---- time series and asynchronous use (request loops) ---- */
 /* every > 0: after every `every`-th step of irbc_md_steps record

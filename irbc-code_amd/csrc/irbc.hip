@@ -25,6 +25,7 @@
 #include <map>
 #include <numeric>
 #include <mutex>
+#include <shared_mutex>
 #include <vector>
 
 #include "../../include/irbc.h"
@@ -37,6 +38,30 @@
 namespace {
 
 thread_local std::string g_err;
+/* Graph captures and stream-ordered allocations of any context on any
+ * thread take turns: a pool allocation or free made while another thread
+ * captures would invalidate that capture. */
+static std::mutex g_capture_mu;
+/* Library calls and graph captures: a capture (inside irbc_md_steps) holds
+ * this exclusively and every other entry point that touches the device
+ * holds it shared, so no call made on another thread (a request loop sets
+ * up contexts on its own threads) runs during a capture.  Nested entry
+ * points take it once per thread. */
+static std::shared_mutex g_api_mu;
+static thread_local int g_api_depth = 0;
+struct ApiGuard {
+    bool held;
+    ApiGuard() : held(g_api_depth++ == 0)
+    {
+        if (held) g_api_mu.lock_shared();
+    }
+    ~ApiGuard()
+    {
+        if (held) g_api_mu.unlock_shared();
+        --g_api_depth;
+    }
+};
+#define API_GUARD ApiGuard api_guard_
 
 int fail(int code, const char *fmt, ...)
 {
@@ -282,8 +307,15 @@ struct irbc_ctx {
         p = nullptr;
         if (n <= 0) n = 1;
         void *q = nullptr;
-        if (hipMalloc(&q, (size_t)n * sizeof(T)) != hipSuccess)
-            return fail(IRBC_ERR_OOM, "hipMalloc of %lld bytes failed", (long long)(n * sizeof(T)));
+        /* stream-ordered: a context freed while others run waits for its
+         * own stream only (hipFree would wait for the whole device) */
+        hipError_t e;
+        {
+            std::lock_guard<std::mutex> lock(g_capture_mu);
+            e = hipMallocAsync(&q, (size_t)n * sizeof(T), st);
+        }
+        if (e != hipSuccess)
+            return fail(IRBC_ERR_OOM, "hipMallocAsync of %lld bytes failed", (long long)(n * sizeof(T)));
         if (hipMemsetAsync(q, 0, (size_t)n * sizeof(T), st) != hipSuccess)
             return fail(IRBC_ERR_HIP, "hipMemsetAsync failed");
         allocs.push_back(q);
@@ -439,6 +471,7 @@ int64_t irbc_estimate_bytes(const irbc_replica_spec *specs, int n)
 
 int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **out)
 {
+    API_GUARD;
     if (!specs || n <= 0 || !out) return fail(IRBC_ERR_ARG, "irbc_create: bad arguments");
     *out = nullptr;
     for (int r = 0; r < n; ++r) {
@@ -454,6 +487,17 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
         return fail(IRBC_ERR_NODEVICE, "irbc_create: no HIP device");
     if (device < 0 || device >= ndev) return fail(IRBC_ERR_ARG, "irbc_create: no device %d", device);
     HIPCHK(hipSetDevice(device));
+    {
+        /* contexts come and go in a request loop: the device's pool keeps
+         * freed memory for the next one instead of returning it at each
+         * synchronisation */
+        std::lock_guard<std::mutex> lock(g_capture_mu);
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+            uint64_t keep = UINT64_MAX;
+            hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+    }
     irbc_ctx *c = new irbc_ctx();
     c->device = device;
     c->nrep = n;
@@ -575,17 +619,25 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
 
 int irbc_destroy(irbc_ctx *c)
 {
+    API_GUARD;
     if (!c) return IRBC_OK;
     if (c->st) hipStreamSynchronize(c->st);
     if (c->st2) hipStreamSynchronize(c->st2);
-    for (void *p : c->allocs) hipFree(p);
+    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
+    c->graphs.clear();
+    /* back to the device's pool in this context's stream order: nothing
+     * else on the device is waited for */
+    {
+        std::lock_guard<std::mutex> lock(g_capture_mu);
+        for (void *p : c->allocs) hipFreeAsync(p, c->st);
+    }
+    if (c->st) hipStreamSynchronize(c->st);
     for (int k = 0; k < 4; ++k)
         for (auto &pr : c->pend[k]) {
             hipEventDestroy(pr.first);
             hipEventDestroy(pr.second);
         }
     for (hipEvent_t e : c->ev_pool) hipEventDestroy(e);
-    for (auto &kv : c->graphs) hipGraphExecDestroy(kv.second);
     for (hipEvent_t e : {c->ev_pre, c->ev_mem2[0], c->ev_mem2[1], c->ev_str[0], c->ev_str[1]})
         if (e) hipEventDestroy(e);
     if (c->st) hipStreamDestroy(c->st);
@@ -596,6 +648,7 @@ int irbc_destroy(irbc_ctx *c)
 
 int irbc_device_bytes(const irbc_ctx *c, int64_t *bytes)
 {
+    API_GUARD;
     if (!c || !bytes) return fail(IRBC_ERR_ARG, "irbc_device_bytes: bad arguments");
     *bytes = c->bytes;
     return IRBC_OK;
@@ -609,6 +662,7 @@ int irbc_device_bytes(const irbc_ctx *c, int64_t *bytes)
 
 int irbc_sync(irbc_ctx *c)
 {
+    API_GUARD;
     if (!c) return fail(IRBC_ERR_ARG, "irbc_sync: null context");
     int rw = c->wait_all();
     if (rw) return rw;
@@ -633,6 +687,7 @@ int irbc_sync(irbc_ctx *c)
 
 int irbc_set_mesh(irbc_ctx *c, int r, const double *xyz, const int32_t *edges, const int32_t *tris)
 {
+    API_GUARD;
     CHECK_R(c, r);
     if (!xyz || !edges || !tris) return fail(IRBC_ERR_ARG, "irbc_set_mesh: null array");
     HostRep &h = c->hr[r];
@@ -826,6 +881,7 @@ int irbc_set_vertices(irbc_ctx *c, int r, const double *x, const double *y, cons
                       const int32_t *pid, const int32_t *bondi, const int32_t *bondl,
                       const int32_t *knob)
 {
+    API_GUARD;
     CHECK_R(c, r);
     const DevRep &R = c->rep[r];
     const size_t n = c->hr[r].spec.n_vertices;
@@ -847,6 +903,7 @@ int irbc_get_vertices(irbc_ctx *c, int r, double *x, double *y, double *z, doubl
                       int32_t *ix, int32_t *iy, int32_t *pid, int32_t *bondi, int32_t *bondl,
                       int32_t *knob)
 {
+    API_GUARD;
     CHECK_R(c, r);
     const DevRep &R = c->rep[r];
     const size_t n = c->hr[r].spec.n_vertices;
@@ -889,6 +946,7 @@ int irbc_set_solvent(irbc_ctx *c, int r, int64_t n, const double *x, const doubl
                      const double *z, const double *vx, const double *vy, const double *vz,
                      const int32_t *indio)
 {
+    API_GUARD;
     CHECK_R(c, r);
     if (n != c->hr[r].spec.n_solvent || !x || !y || !z || !vx || !vy || !vz)
         return fail(IRBC_ERR_ARG, "irbc_set_solvent: expected %lld particles", (long long)c->hr[r].spec.n_solvent);
@@ -930,6 +988,7 @@ static int launch_scan(irbc_ctx *c, int staged, int which, int32_t *in, int32_t 
 
 int irbc_init_solvent(irbc_ctx *c, int r, int maxwell)
 {
+    API_GUARD;
     CHECK_R(c, r);
     int rc = c->pull(r);
     if (rc) return rc;
@@ -954,6 +1013,7 @@ int irbc_init_solvent(irbc_ctx *c, int r, int maxwell)
 int irbc_get_solvent(irbc_ctx *c, int r, double *x, double *y, double *z, double *vx, double *vy,
                      double *vz, int32_t *key, int32_t *indio)
 {
+    API_GUARD;
     CHECK_R(c, r);
     const int64_t n = c->hr[r].spec.n_solvent;
     HIPCHK(c->sync_both());
@@ -976,6 +1036,7 @@ int irbc_get_solvent(irbc_ctx *c, int r, double *x, double *y, double *z, double
 
 int irbc_set_parasite(irbc_ctx *c, int r, const double *d, const int32_t *i)
 {
+    API_GUARD;
     CHECK_R(c, r);
     if (!d || !i) return fail(IRBC_ERR_ARG, "irbc_set_parasite: null array");
     int rc = c->pull(r);
@@ -995,6 +1056,7 @@ int irbc_set_parasite(irbc_ctx *c, int r, const double *d, const int32_t *i)
 
 int irbc_get_parasite(irbc_ctx *c, int r, double *d, int32_t *i)
 {
+    API_GUARD;
     CHECK_R(c, r);
     int rc = c->pull(r);
     if (rc) return rc;
@@ -1013,12 +1075,14 @@ int irbc_get_parasite(irbc_ctx *c, int r, double *d, int32_t *i)
 
 int64_t irbc_n_ligands(const irbc_ctx *c, int r)
 {
+    API_GUARD;
     if (!c || r < 0 || r >= c->nrep) return IRBC_ERR_ARG;
     return c->rep[r].nlig;
 }
 
 int irbc_set_ligands(irbc_ctx *c, int r, const int32_t *ligi)
 {
+    API_GUARD;
     CHECK_R(c, r);
     HIPCHK(hipMemcpy(c->ligi + c->rep[r].loff, ligi, (size_t)c->rep[r].nlig * 4, hipMemcpyHostToDevice));
     return IRBC_OK;
@@ -1026,6 +1090,7 @@ int irbc_set_ligands(irbc_ctx *c, int r, const int32_t *ligi)
 
 int irbc_get_ligands(irbc_ctx *c, int r, int32_t *ligi)
 {
+    API_GUARD;
     CHECK_R(c, r);
     HIPCHK(c->sync_both());
     HIPCHK(hipMemcpy(ligi, c->ligi + c->rep[r].loff, (size_t)c->rep[r].nlig * 4, hipMemcpyDeviceToHost));
@@ -1034,6 +1099,7 @@ int irbc_get_ligands(irbc_ctx *c, int r, int32_t *ligi)
 
 int irbc_get_spec(const irbc_ctx *c, int r, irbc_replica_spec *out)
 {
+    API_GUARD;
     CHECK_R(c, r);
     if (!out) return fail(IRBC_ERR_ARG, "irbc_get_spec: null output");
     *out = c->hr[r].spec;
@@ -1042,6 +1108,7 @@ int irbc_get_spec(const irbc_ctx *c, int r, irbc_replica_spec *out)
 
 int irbc_get_globals(irbc_ctx *c, int r, double *g)
 {
+    API_GUARD;
     CHECK_R(c, r);
     int rc = c->pull(r);
     if (rc) return rc;
@@ -1064,6 +1131,7 @@ int irbc_get_globals(irbc_ctx *c, int r, double *g)
 
 int irbc_set_counters(irbc_ctx *c, int r, int64_t step, int32_t colsteps, int32_t upd)
 {
+    API_GUARD;
     CHECK_R(c, r);
     if (!counters_valid(c->hr[r].spec.params, step, colsteps, upd))
         return fail(IRBC_ERR_ARG, "irbc_set_counters: need step >= 0, -1 <= colsteps < csteps, -1 <= upd < uplist");
@@ -1079,6 +1147,7 @@ int irbc_set_counters(irbc_ctx *c, int r, int64_t step, int32_t colsteps, int32_
 
 int irbc_get_counters(irbc_ctx *c, int r, int64_t *step, int32_t *colsteps, int32_t *upd)
 {
+    API_GUARD;
     CHECK_R(c, r);
     *step = c->hr[r].step;
     *colsteps = c->hr[r].colsteps;
@@ -1088,6 +1157,7 @@ int irbc_get_counters(irbc_ctx *c, int r, int64_t *step, int32_t *colsteps, int3
 
 int irbc_set_wallspeed(irbc_ctx *c, int r, double w)
 {
+    API_GUARD;
     CHECK_R(c, r);
     int rc = c->pull(r);
     if (rc) return rc;
@@ -1097,6 +1167,7 @@ int irbc_set_wallspeed(irbc_ctx *c, int r, double w)
 
 int irbc_set_seed(irbc_ctx *c, int r, uint64_t seed)
 {
+    API_GUARD;
     CHECK_R(c, r);
     int rc = c->pull(r);
     if (rc) return rc;
@@ -1107,6 +1178,7 @@ int irbc_set_seed(irbc_ctx *c, int r, uint64_t seed)
 
 int irbc_set_series(irbc_ctx *c, int r, int32_t every, int32_t cap)
 {
+    API_GUARD;
     CHECK_R(c, r);
     if (every < 0 || cap < 0 || (every > 0 && cap == 0))
         return fail(IRBC_ERR_ARG, "irbc_set_series: every >= 0 and cap >= 1 when on");
@@ -1127,6 +1199,7 @@ int irbc_set_series(irbc_ctx *c, int r, int32_t every, int32_t cap)
 
 int64_t irbc_get_series(irbc_ctx *c, int r, double *out, int64_t cap)
 {
+    API_GUARD;
     CHECK_R(c, r);
     int rc = c->pull(r);
     if (rc) return rc;
@@ -1140,6 +1213,7 @@ int64_t irbc_get_series(irbc_ctx *c, int r, double *out, int64_t cap)
 
 int irbc_query(irbc_ctx *c)
 {
+    API_GUARD;
     if (!c) return fail(IRBC_ERR_ARG, "irbc_query: null context");
     hipStream_t ss[2] = {c->st, c->st2};
     for (hipStream_t q : ss) {
@@ -1152,6 +1226,7 @@ int irbc_query(irbc_ctx *c)
 
 int irbc_use_tapes(irbc_ctx *c, int r, int on)
 {
+    API_GUARD;
     CHECK_R(c, r);
     int rc = c->pull(r);
     if (rc) return rc;
@@ -1162,6 +1237,7 @@ int irbc_use_tapes(irbc_ctx *c, int r, int on)
 int irbc_set_collision_tape(irbc_ctx *c, int r, const double *shift_u, const double *cell_u,
                             const double *wall_z, const double *mem_z)
 {
+    API_GUARD;
     CHECK_R(c, r);
     if (!shift_u || !cell_u || !wall_z || !mem_z) return fail(IRBC_ERR_ARG, "irbc_set_collision_tape: null tape");
     int rc = c->pull(r);
@@ -1178,6 +1254,7 @@ int irbc_set_collision_tape(irbc_ctx *c, int r, const double *shift_u, const dou
 
 int irbc_set_bond_tape(irbc_ctx *c, int r, const double *u, int64_t n)
 {
+    API_GUARD;
     CHECK_R(c, r);
     const DevRep &R = c->rep[r];
     if (n < 0 || n > (int64_t)R.nv * IRBC_MAX_CAND) return fail(IRBC_ERR_ARG, "irbc_set_bond_tape: too many draws");
@@ -1472,6 +1549,7 @@ static int ready(irbc_ctx *c, bool need_solvent)
 
 int irbc_collision_step(irbc_ctx *c)
 {
+    API_GUARD;
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     c->primed = false;
     int rc = ready(c, true);
@@ -1483,6 +1561,7 @@ int irbc_collision_step(irbc_ctx *c)
 
 int irbc_llist_update(irbc_ctx *c)
 {
+    API_GUARD;
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     c->primed = false;
     int rc = ready(c, false);
@@ -1494,6 +1573,7 @@ int irbc_llist_update(irbc_ctx *c)
 
 int irbc_stream(irbc_ctx *c)
 {
+    API_GUARD;
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     c->primed = false;
     int rc = ready(c, true);
@@ -1511,6 +1591,7 @@ int irbc_stream(irbc_ctx *c)
 
 int irbc_classify_solvent(irbc_ctx *c)
 {
+    API_GUARD;
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     c->primed = false;
     int rc = ready(c, true);
@@ -1532,15 +1613,22 @@ int irbc_classify_solvent(irbc_ctx *c)
     }
     int64_t *d_soff = nullptr, *d_eoff = nullptr;
     int32_t *d_cnt = nullptr, *d_start = nullptr, *d_ent = nullptr;
+    /* stream-ordered, so that set-up beside a running context never waits
+     * for that context's work */
     auto release = [&]() {
-        hipFree(d_soff);
-        hipFree(d_eoff);
-        hipFree(d_cnt);
-        hipFree(d_start);
-        hipFree(d_ent);
+        {
+            std::lock_guard<std::mutex> lock(g_capture_mu);
+            for (void *p : {(void *)d_soff, (void *)d_eoff, (void *)d_cnt, (void *)d_start, (void *)d_ent})
+                if (p) hipFreeAsync(p, c->st);
+        }
+        hipStreamSynchronize(c->st);
     };
-    if (hipMalloc(&d_soff, n * 8) != hipSuccess || hipMalloc(&d_eoff, n * 8) != hipSuccess ||
-        hipMalloc(&d_cnt, nbt * 4) != hipSuccess || hipMalloc(&d_start, (nbt + n) * 4) != hipSuccess) {
+    auto dmalloc = [&](void **p, size_t bytes) {
+        std::lock_guard<std::mutex> lock(g_capture_mu);
+        return hipMallocAsync(p, bytes, c->st);
+    };
+    if (dmalloc((void **)&d_soff, n * 8) != hipSuccess || dmalloc((void **)&d_eoff, n * 8) != hipSuccess ||
+        dmalloc((void **)&d_cnt, nbt * 4) != hipSuccess || dmalloc((void **)&d_start, (nbt + n) * 4) != hipSuccess) {
         release();
         return fail(IRBC_ERR_OOM, "irbc_classify_solvent: no room for the triangle bins");
     }
@@ -1548,14 +1636,15 @@ int irbc_classify_solvent(irbc_ctx *c)
     std::vector<int32_t> start(nbt + n);
     const dim3 g((unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG), (unsigned)n);
     const MemView V = c->view(c->par);
-    if (hipMemcpy(d_soff, soff.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess ||
+    if (hipMemcpyAsync(d_soff, soff.data(), n * 8, hipMemcpyHostToDevice, c->st) != hipSuccess ||
         hipMemsetAsync(d_cnt, 0, nbt * 4, c->st) != hipSuccess)
         rc = fail(IRBC_ERR_HIP, "irbc_classify_solvent: set-up copies failed");
     if (!rc) {
         k_cls_bins<<<g, IRBC_WG, 0, c->st>>>(c->d_rep, V, 0, d_soff, d_cnt, nullptr, nullptr, nullptr);
         k_cls_scan<<<(unsigned)n, IRBC_WG, 0, c->st>>>(c->d_rep, d_soff, d_cnt, d_start);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->st) != hipSuccess ||
-            hipMemcpy(start.data(), d_start, (nbt + n) * 4, hipMemcpyDeviceToHost) != hipSuccess)
+            hipMemcpyAsync(start.data(), d_start, (nbt + n) * 4, hipMemcpyDeviceToHost, c->st) != hipSuccess ||
+            hipStreamSynchronize(c->st) != hipSuccess)
             rc = fail(IRBC_ERR_HIP, "irbc_classify_solvent: counting the bins failed");
     }
     int64_t tot = 0;
@@ -1563,9 +1652,9 @@ int irbc_classify_solvent(irbc_ctx *c)
         eoff[r] = tot;
         tot += start[soff[r] + r + (int64_t)c->rep[r].Ly * (c->rep[r].Lz + 2)];
     }
-    if (!rc && (hipMalloc(&d_ent, std::max<int64_t>(tot, 1) * 4) != hipSuccess))
+    if (!rc && (dmalloc((void **)&d_ent, std::max<int64_t>(tot, 1) * 4) != hipSuccess))
         rc = fail(IRBC_ERR_OOM, "irbc_classify_solvent: no room for the bin entries");
-    if (!rc && hipMemcpy(d_eoff, eoff.data(), n * 8, hipMemcpyHostToDevice) != hipSuccess)
+    if (!rc && hipMemcpyAsync(d_eoff, eoff.data(), n * 8, hipMemcpyHostToDevice, c->st) != hipSuccess)
         rc = fail(IRBC_ERR_HIP, "irbc_classify_solvent: set-up copies failed");
     if (!rc) {
         k_cls_bins<<<g, IRBC_WG, 0, c->st>>>(c->d_rep, V, 1, d_soff, d_cnt, d_start, d_eoff, d_ent);
@@ -1581,6 +1670,7 @@ int irbc_classify_solvent(irbc_ctx *c)
 
 int irbc_membrane_phases(irbc_ctx *c, int phases)
 {
+    API_GUARD;
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     if (phases & ~(255 | IRBC_PH_PARASITE | IRBC_PH_MEMBRANE)) return fail(IRBC_ERR_ARG, "unknown phase bits 0x%x", phases);
     /* the public force bits onto the kernels' two force phases */
@@ -1601,6 +1691,7 @@ int irbc_membrane_phases(irbc_ctx *c, int phases)
 
 int irbc_membrane_va(irbc_ctx *c, int r, double *area, double *volume)
 {
+    API_GUARD;
     CHECK_R(c, r);
     int rc = irbc_membrane_phases(c, PH_VA);
     if (rc) return rc;
@@ -1616,6 +1707,7 @@ int irbc_bond_step(irbc_ctx *c) { return irbc_membrane_phases(c, PH_BONDS); }
 int irbc_get_cells(irbc_ctx *c, int r, double *vcm, int32_t *nc, double *ncc, double *rand1,
                    double *rand2, double *dvv, double *the)
 {
+    API_GUARD;
     CHECK_R(c, r);
     const DevRep &R = c->rep[r];
     const size_t n = R.ncell;
@@ -1631,6 +1723,7 @@ int irbc_get_cells(irbc_ctx *c, int r, double *vcm, int32_t *nc, double *ncc, do
 
 int64_t irbc_get_llist(irbc_ctx *c, int r, int64_t *off, int32_t *ent, int64_t cap)
 {
+    API_GUARD;
     CHECK_R(c, r);
     const DevRep &R = c->rep[r];
     const size_t n = R.ncell;
@@ -1646,6 +1739,7 @@ int64_t irbc_get_llist(irbc_ctx *c, int r, int64_t *off, int32_t *ent, int64_t c
 
 int64_t irbc_get_pairs(irbc_ctx *c, int r, int32_t *cnt, int32_t *lig, int64_t cap)
 {
+    API_GUARD;
     CHECK_R(c, r);
     const DevRep &R = c->rep[r];
     const int nv = R.nv;
@@ -1838,6 +1932,7 @@ static int64_t llist_len(irbc_ctx *c, int r)
 
 int64_t irbc_checkpoint_bytes(irbc_ctx *c, int r)
 {
+    API_GUARD;
     CHECK_R(c, r);
     HIPCHK(c->sync_both());
     const int64_t llen = llist_len(c, r);
@@ -1847,6 +1942,7 @@ int64_t irbc_checkpoint_bytes(irbc_ctx *c, int r)
 
 int64_t irbc_checkpoint_save(irbc_ctx *c, int r, void *buf, int64_t cap)
 {
+    API_GUARD;
     CHECK_R(c, r);
     if (!buf) return fail(IRBC_ERR_ARG, "irbc_checkpoint_save: null buffer");
     int rc = c->pull(r);
@@ -1982,6 +2078,7 @@ static int ckpt_check(const irbc_ctx *c, int r, const CkptHeader &H, const char 
 
 int irbc_checkpoint_load(irbc_ctx *c, int r, const void *buf, int64_t n)
 {
+    API_GUARD;
     CHECK_R(c, r);
     if (!buf || n < (int64_t)sizeof(CkptHeader)) return fail(IRBC_ERR_ARG, "irbc_checkpoint_load: short buffer");
     CkptHeader H;
@@ -2059,6 +2156,139 @@ int irbc_checkpoint_load(irbc_ctx *c, int r, const void *buf, int64_t n)
     top_of_step(hm.spec.params, H.step, H.colsteps, H.upd, R);
     R.wall2[0] = R.wall2[1] = H.wallspeed;
     return c->push(r);
+}
+
+/* A replica moved device to device: src's replica rs into dst's slot rd,
+ * with everything a checkpoint holds, its mesh (topology, rest lengths,
+ * rest areas, hinge orientation, force-slot incidence, A0, V0, theta0) and
+ * its time series (cadence, capacity, records).  Same sizes and parameters
+ * as a checkpoint load requires; dst needs no irbc_set_mesh first.  dst
+ * then continues as src would have.  A request loop uses it to re-pack the
+ * running replicas into a new context when requests arrive or finish;
+ * nothing crosses the host but the descriptors and two list offsets. */
+int irbc_copy_replica(irbc_ctx *dst, int rd, irbc_ctx *src, int rs)
+{
+    API_GUARD;
+    CHECK_R(dst, rd);
+    CHECK_R(src, rs);
+    if (dst == src && rd == rs) return IRBC_OK;
+    if (dst->device != src->device) return fail(IRBC_ERR_ARG, "irbc_copy_replica: contexts on different devices");
+    const HostRep &hs = src->hr[rs];
+    HostRep &hd = dst->hr[rd];
+    if (!hs.mesh || !hs.solvent) return fail(IRBC_ERR_STATE, "irbc_copy_replica: the source replica is not set up");
+    const irbc_replica_spec &a = hs.spec, &b = hd.spec;
+    if (a.n_solvent != b.n_solvent || a.n_vertices != b.n_vertices || a.n_edges != b.n_edges ||
+        a.n_triangles != b.n_triangles || src->rep[rs].ncell != dst->rep[rd].ncell ||
+        src->rep[rs].nlig != dst->rep[rd].nlig)
+        return fail(IRBC_ERR_ARG, "irbc_copy_replica: the replicas have other sizes");
+    if (!same_params(a.params, b.params))
+        return fail(IRBC_ERR_ARG, "irbc_copy_replica: the replicas have other parameters");
+    int rc = src->pull(rs);
+    if (rc) return rc;
+    rc = dst->pull(rd);
+    if (rc) return rc;
+    const DevRep &S = src->rep[rs];
+    DevRep &R = dst->rep[rd];
+    if (S.rotate_pending) return fail(IRBC_ERR_STATE, "irbc_copy_replica: a collision's rotation is pending");
+    const int64_t ns = a.n_solvent, nv = a.n_vertices, ne = a.n_edges, nt = a.n_triangles;
+    const int64_t nlig = S.nlig, ncell = S.ncell;
+    if (S.series_every > 0 && R.series_cap < S.series_cap) {
+        double *buf = nullptr;
+        rc = dst->alloc(buf, (int64_t)S.series_cap * IRBC_SERIES_REC);
+        if (rc) return rc;
+        R.series = buf; /* freed with the context */
+        R.series_cap = S.series_cap;
+    }
+    int32_t ends[2] = {0, 0};
+    HIPCHK(hipMemcpy(&ends[0], src->lloff + S.coff1, 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&ends[1], src->lloff + S.coff1 + ncell, 4, hipMemcpyDeviceToHost));
+    const int64_t llen = (int64_t)ends[1] - ends[0];
+    if (ends[0] < 0 || llen < 0 || llen > R.llent_cap)
+        return fail(IRBC_ERR_ARG, "irbc_copy_replica: cell lists exceed the destination's capacity");
+    hipStream_t q = dst->st;
+#define D2D(d_, s_, bytes_) HIPCHK(hipMemcpyAsync((d_), (s_), (size_t)(bytes_), hipMemcpyDeviceToDevice, q))
+    /* solvent */
+    float *const df[] = {dst->xa, dst->ya, dst->za, dst->vxa, dst->vya, dst->vza};
+    const float *const sf[] = {src->xa, src->ya, src->za, src->vxa, src->vya, src->vza};
+    for (int k = 0; k < 6; ++k) D2D(df[k] + R.poff, sf[k] + S.poff, ns * 4);
+    D2D(dst->ida + R.poff, src->ida + S.poff, ns * 4);
+    /* vertices, bonds and knobs */
+    double *const dd[] = {dst->mx, dst->my, dst->mz, dst->mvx, dst->mvy, dst->mvz,
+                          dst->mfx, dst->mfy, dst->mfz, dst->poff, dst->pon};
+    const double *const sd[] = {src->mx, src->my, src->mz, src->mvx, src->mvy, src->mvz,
+                                src->mfx, src->mfy, src->mfz, src->poff, src->pon};
+    for (int k = 0; k < 11; ++k) D2D(dd[k] + R.voff, sd[k] + S.voff, nv * 8);
+    int32_t *const di[] = {dst->mix, dst->miy, dst->mpid, dst->bondi, dst->bondl, dst->knob, dst->ncand};
+    const int32_t *const si[] = {src->mix, src->miy, src->mpid, src->bondi, src->bondl, src->knob, src->ncand};
+    for (int k = 0; k < 7; ++k) D2D(di[k] + R.voff, si[k] + S.voff, nv * 4);
+    if (nlig > 0) D2D(dst->ligi + R.loff, src->ligi + S.loff, nlig * 4);
+    /* the mesh: every index in these arrays is the replica's own */
+    int32_t *const die[] = {dst->e1, dst->e2, dst->es1, dst->es2, dst->iedge};
+    const int32_t *const sie[] = {src->e1, src->e2, src->es1, src->es2, src->iedge};
+    for (int k = 0; k < 5; ++k) D2D(die[k] + R.eoff, sie[k] + S.eoff, ne * 4);
+    D2D(dst->l0 + R.eoff, src->l0 + S.eoff, ne * 8);
+    D2D(dst->kappap + R.eoff, src->kappap + S.eoff, ne * 8);
+    int32_t *const dit[] = {dst->t1, dst->t2, dst->t3};
+    const int32_t *const sit[] = {src->t1, src->t2, src->t3};
+    for (int k = 0; k < 3; ++k) D2D(dit[k] + R.toff, sit[k] + S.toff, nt * 4);
+    D2D(dst->tarea + R.toff, src->tarea + S.toff, nt * 8);
+    D2D(dst->inc_off + R.inc_off, src->inc_off + S.inc_off, (nv + 1) * 4);
+    D2D(dst->inc_ent + R.inc_ent_off, src->inc_ent + S.inc_ent_off, inc_bound(a) * 4);
+    D2D(dst->tag + R.scr_off, src->tag + S.scr_off, scr_bound(a) * 8);
+    /* the cell->triangle lists of the last update */
+    if (ends[0] == 0) {
+        D2D(dst->lloff + R.coff1, src->lloff + S.coff1, (ncell + 1) * 4);
+    } else {
+        /* offsets relative to the replica's first entry */
+        std::vector<int32_t> off(ncell + 1);
+        HIPCHK(hipMemcpy(off.data(), src->lloff + S.coff1, (ncell + 1) * 4, hipMemcpyDeviceToHost));
+        for (int32_t &o : off) o -= ends[0];
+        HIPCHK(hipMemcpyAsync(dst->lloff + R.coff1, off.data(), (ncell + 1) * 4, hipMemcpyHostToDevice, q));
+        HIPCHK(hipStreamSynchronize(q));
+    }
+    if (llen > 0) D2D(dst->llent + R.llent_off, src->llent + S.llent_off + ends[0], llen * 4);
+    if (S.series_every > 0 && S.series_n > 0)
+        D2D(R.series, S.series, (int64_t)std::min(S.series_n, S.series_cap) * IRBC_SERIES_REC * 8);
+    /* the particle order of the source, no pending rotation */
+    HIPCHK(hipMemsetAsync(R.rec + 4 * R.poff, 0, (size_t)ns * 16, q));
+    HIPCHK(hipMemsetAsync(R.rec1 + 4 * R.poff, 0, (size_t)ns * 16, q));
+#undef D2D
+    R.Sarea0 = S.Sarea0;
+    R.Volume0 = S.Volume0;
+    R.theta0 = S.theta0;
+    R.cos_t0 = S.cos_t0;
+    R.sin_t0 = S.sin_t0;
+    for (int k = 0; k < 3; ++k) {
+        R.para[k] = S.para[k];
+        R.paraV[k] = S.paraV[k];
+        R.paraft[k] = S.paraft[k];
+        R.parafdt[k] = S.parafdt[k];
+    }
+    R.paradx = S.paradx;
+    R.parady = S.parady;
+    R.paraid = S.paraid;
+    R.rotate_pending = 0;
+    R.key_src = 1;
+    R.seed = S.seed;
+    R.tapes = S.tapes;
+    R.status = 0;
+    R.near_count[0] = R.near_count[1] = R.defer_count[0] = R.defer_count[1] = 0;
+    R.series_every = S.series_every;
+    R.series_n = S.series_every > 0 ? S.series_n : 0;
+    hd.e1 = hs.e1;
+    hd.e2 = hs.e2;
+    hd.t1 = hs.t1;
+    hd.t2 = hs.t2;
+    hd.t3 = hs.t3;
+    hd.mesh = true;
+    hd.spec.seed = hs.spec.seed;
+    hd.step = hs.step;
+    hd.colsteps = hs.colsteps;
+    hd.upd = hs.upd;
+    hd.solvent = true;
+    top_of_step(hd.spec.params, hs.step, hs.colsteps, hs.upd, R);
+    R.wall2[0] = R.wall2[1] = S.wall2[src->par & 1];
+    return dst->push(rd);
 }
 
 /* ---------------- the time loop ---------------- */
@@ -2371,6 +2601,9 @@ static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
         const int par0 = c->par;
+        if (g_api_depth > 0) return fail(IRBC_ERR_STATE, "graph capture inside another library call");
+        std::unique_lock<std::shared_mutex> api_lock(g_api_mu);
+        std::lock_guard<std::mutex> lock(g_capture_mu);
         HIPCHK(hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
         int rc = IRBC_OK;
         for (int k = 0; k < n && rc == IRBC_OK; ++k) {
@@ -2475,6 +2708,7 @@ extern "C" int irbc_debug_chain_clock(irbc_ctx *c, long long *out)
 
 int irbc_set_option(irbc_ctx *c, int option, int value)
 {
+    API_GUARD;
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     c->primed = false; /* the masks' and queues' use differs between shapes */
     switch (option) {
@@ -2550,6 +2784,7 @@ int irbc_set_option(irbc_ctx *c, int option, int value)
 
 int irbc_set_diagnostics(irbc_ctx *c, int on)
 {
+    API_GUARD;
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     c->diag_on = on != 0;
     return IRBC_OK;
@@ -2557,6 +2792,7 @@ int irbc_set_diagnostics(irbc_ctx *c, int on)
 
 int irbc_profile(irbc_ctx *c, int on)
 {
+    API_GUARD;
     if (!c) return fail(IRBC_ERR_ARG, "null context");
     int rc = c->flush_prof();
     if (rc) return rc;
@@ -2572,6 +2808,7 @@ int irbc_profile(irbc_ctx *c, int on)
 
 int irbc_profile_read(irbc_ctx *c, double *t)
 {
+    API_GUARD;
     if (!c || !t) return fail(IRBC_ERR_ARG, "irbc_profile_read: bad arguments");
     int rc = c->flush_prof();
     if (rc) return rc;
@@ -2647,6 +2884,7 @@ int irbc_membrane_va_static(const double *x, const double *y, const double *z, c
                             const int32_t *iy, int32_t nv, const int32_t *tris, int32_t nt, double Lx, double Ly,
                             double *out, void *stream)
 {
+    API_GUARD;
     if (!x || !y || !z || !ix || !iy || !tris || !out || nv < 3 || nt < 1)
         return fail(IRBC_ERR_ARG, "irbc_membrane_va_static: null array or empty mesh");
     const int nblk = (nt + IRBC_WG - 1) / IRBC_WG;

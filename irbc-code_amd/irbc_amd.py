@@ -28,7 +28,7 @@ EXPORTS = (
     "irbc_get_pairs", "irbc_md_steps", "irbc_set_diagnostics", "irbc_profile",
     "irbc_profile_read", "irbc_set_series", "irbc_get_series", "irbc_query", "irbc_set_option",
     "irbc_checkpoint_bytes", "irbc_checkpoint_save", "irbc_checkpoint_load", "irbc_autotune",
-    "irbc_membrane_va_static", "irbc_init_solvent", "irbc_get_spec",
+    "irbc_membrane_va_static", "irbc_init_solvent", "irbc_get_spec", "irbc_copy_replica",
 )
 
 OPT_MEMBRANE_FUSED = 1
@@ -156,6 +156,7 @@ def load():
     L.irbc_checkpoint_save.restype = C.c_int64
     L.irbc_checkpoint_save.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
     L.irbc_checkpoint_load.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
+    L.irbc_copy_replica.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
     L.irbc_membrane_va_static.argtypes = [C.c_void_p] * 5 + [C.c_int32, C.c_void_p, C.c_int32, C.c_double,
                                                              C.c_double, C.c_void_p, C.c_void_p]
     _lib = L
@@ -425,6 +426,11 @@ class Batch:
         """Load a checkpoint into replica r (its mesh must be set)."""
         buf = C.create_string_buffer(bytes(data), len(data))
         _chk(self.L.irbc_checkpoint_load(self.h, r, buf, len(data)))
+
+    def copy_from(self, r, src, rs):
+        """Replica rs of batch `src` into slot r of this batch, device to
+        device: state, mesh and time series (same sizes and parameters)."""
+        _chk(self.L.irbc_copy_replica(self.h, int(r), src.h, int(rs)))
 
     def autotune(self, steps=20):
         """Pick the fastest launch options for this batch; returns

@@ -5,6 +5,8 @@ The reference reads a triangulated cell from three files and places it
 ellipsoids of 162 / 642 / 2562 vertices.  Everything is numpy on the host;
 the state is then handed to the device through the C ABI.
 """
+import functools
+
 import numpy as np
 
 
@@ -53,12 +55,19 @@ def edges_of(tris):
     return np.array(list(seen.values()), dtype=np.int32)
 
 
+@functools.lru_cache(maxsize=None)
 def mesh_for(nv):
+    """The unit icosphere of nv vertices: positions, edges, triangles.  Built
+    once per size (a request loop sets up many replicas a second) and
+    returned read-only: callers scale and place copies."""
     sub = {12: 0, 42: 1, 162: 2, 642: 3, 2562: 4}
     if nv not in sub:
         raise ValueError("icosphere sizes are 12, 42, 162, 642, 2562 vertices")
     V, T = icosphere(sub[nv])
-    return V, edges_of(T), T
+    E = edges_of(T)
+    for a in (V, E, T):
+        a.setflags(write=False)
+    return V, E, T
 
 
 def place_reference(V, Lx, Ly):

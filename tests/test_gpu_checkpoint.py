@@ -97,3 +97,63 @@ def test_counters_out_of_range_are_refused():
         b.set_counters(0, 0, p.csteps - 1, p.uplist - 1)
     finally:
         b.close()
+
+
+@pytest.mark.gpu
+def test_copy_replica_device_to_device_is_bit_exact():
+    """irbc_copy_replica moves a running replica into a slot of another
+    context without a host round trip of its state; the copy continues bit
+    for bit, time series included, and the source is left as it was.  The
+    first slot had its mesh and series set beforehand, the others take
+    them from the source."""
+    ds = _mix()
+    b = en.batch(ds)
+    for r in range(len(ds)):
+        b.set_series(r, 10, 16)
+    b.md_steps(33)  # mid collision period, between list updates
+    b.sync()
+    order = [1, 2, 0]
+    b2 = ia.Batch([en.spec(ds[i]) for i in order])
+    try:
+        for slot, i in enumerate(order):
+            if slot == 0:
+                V, E, T = en.mesh(ds[i])
+                b2.set_mesh(slot, V, E, T)
+                b2.set_series(slot, 10, 16)
+            b2.copy_from(slot, b, i)
+        b.md_steps(40)
+        b2.md_steps(40)
+        b.sync()
+        b2.sync()
+        for slot, i in enumerate(order):
+            want, got = _snapshot(b, i), _snapshot(b2, slot)
+            assert len(got["series"]) == 7
+            for k in want:
+                np.testing.assert_array_equal(got[k], want[k], err_msg="replica %d %s" % (i, k))
+    finally:
+        b2.close()
+        b.close()
+
+
+@pytest.mark.gpu
+def test_copy_replica_refuses_other_sizes_and_parameters():
+    ds = _mix()
+    d1 = dict(ds[0], shear=ds[0]["shear"] * 0.7)  # same sizes, other parameters
+    b = en.batch([ds[0], ds[1], d1])
+    b2 = ia.Batch([en.spec(ds[0])])
+    try:
+        b.set_series(0, 10, 4)
+        b.md_steps(12)
+        b.sync()
+        with pytest.raises(ia.IrbcError, match="other sizes"):
+            b2.copy_from(0, b, 1)
+        with pytest.raises(ia.IrbcError, match="other parameters"):
+            b2.copy_from(0, b, 2)
+        with pytest.raises(ia.IrbcError, match="not set up"):
+            b.copy_from(0, b2, 0)
+        b2.copy_from(0, b, 0)
+        assert len(b2.series(0)) == 1
+        assert b2.counters(0) == b.counters(0)
+    finally:
+        b2.close()
+        b.close()

@@ -175,18 +175,19 @@ def test_uneven_48_replica_batch():
     _batch_case(ds, sample_oracle=(m2562, m162), sample_alone=(big, 47))
 
 
-@pytest.mark.parametrize("options", [None, 12])
-def test_poisson_stream_of_uneven_requests(options):
+@pytest.mark.parametrize("options,repack", [(None, False), (12, False), (12, True)])
+def test_poisson_stream_of_uneven_requests(options, repack):
     """Requests drawn like the uneven batch, Poisson arrivals faster than
     they run, served under a budget that makes some wait; sampled results
-    equal the same requests served alone.  With options 12 the micro-batches
-    of several replicas run the membrane on a second stream as grid kernels,
+    equal the same requests served alone.  With options 12 contexts of
+    several replicas run the membrane on a second stream as grid kernels,
     with XCD chunking and captured graphs (the shape the autotuner picks for
     the uneven batch), while contexts are built and closed on the builder
-    thread."""
+    thread.  Micro-batches (repack False) run side by side; re-packing moves
+    the running requests into each new context device to device."""
     reqs = serve.poisson_requests(12, 200.0, 2000, seed=5, series_every=100)
-    budget = int(0.5 * sum(r.bytes for r in reqs))
-    srv = serve.Server(budget, chunk=100, max_active=3, window=0.005, options=options)
+    budget = int((2.0 if repack else 0.5) * sum(r.bytes for r in reqs))
+    srv = serve.Server(budget, chunk=100, max_active=3, window=0.005, options=options, repack=repack)
     srv.run(reqs)
     assert all(r.finished is not None for r in reqs)
     assert srv.used == 0
