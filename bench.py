@@ -264,7 +264,7 @@ def plain_rate(ds, steps, warmup=50):
 
 
 def served(n, rate, steps, budget_gb, plain_every=8, plain_steps=1000, options=None, repack=True, chunk=250,
-           window=0.1):
+           window=0.1, repack_idle=0.25):
     """Poisson arrivals of uneven-ensemble requests through the request
     loop, against the plain path on every plain_every-th request (a sample
     of the same mix, run one after another); contexts of several replicas
@@ -273,7 +273,7 @@ def served(n, rate, steps, budget_gb, plain_every=8, plain_steps=1000, options=N
     and finishes; else micro-batches side by side."""
     reqs = serve.poisson_requests(n, rate, steps, seed=1)
     srv = serve.Server(int(budget_gb * 2 ** 30), chunk=chunk, max_active=2, window=window, options=options,
-                       repack=repack)
+                       repack=repack, repack_idle=repack_idle)
     srv.run(reqs)
     out = serve.summary(reqs, srv.makespan)
     sample = [r.replica for r in reqs[::plain_every]]
@@ -282,6 +282,7 @@ def served(n, rate, steps, budget_gb, plain_every=8, plain_steps=1000, options=N
                 "schedule": "re-packing: one running context, rebuilt at admissions and finishes" if repack
                 else "micro-batches side by side",
                 "hbm_budget_GiB": budget_gb, "chunk_steps": srv.chunk, "batching_window_s": srv.window,
+                "repack_idle_share": srv.repack_idle if repack else None,
                 "contexts_built": srv.stats["groups"], "max_concurrent_contexts": srv.stats["max_active_groups"],
                 "max_resident_replicas": srv.stats["max_resident_replicas"],
                 "launch_options_index": options,
@@ -377,6 +378,8 @@ def main():
     ap.add_argument("--served-chunk", type=int, default=250)
     ap.add_argument("--served-window", type=float, default=0.1)
     ap.add_argument("--served-budget-gb", type=float, default=64.0)
+    ap.add_argument("--served-idle", type=float, default=0.25,
+                    help="re-pack once finished requests hold this share of the running context's particles")
     ap.add_argument("--served-microbatches", action="store_true",
                     help="serve with micro-batches side by side instead of re-packing")
     ap.add_argument("--profile-every", type=int, default=4)
@@ -405,7 +408,7 @@ def main():
         print(json.dumps({"served": served(a.served_requests, a.served_rate, a.served_steps, a.served_budget_gb,
                                            options=None if a.options < 0 else a.options,
                                            repack=not a.served_microbatches, chunk=a.served_chunk,
-                                           window=a.served_window)}))
+                                           window=a.served_window, repack_idle=a.served_idle)}))
         return
     d = en.single()
     b = ia.Batch([en.spec(d)])
@@ -480,7 +483,8 @@ def main():
         # autotuner picked for the uneven ensemble, the same mix
         opt = line["uneven"]["launch_options_index"] if "uneven" in line else None
         line["served"] = served(a.served_requests, a.served_rate, a.served_steps, a.served_budget_gb, options=opt,
-                                repack=not a.served_microbatches, chunk=a.served_chunk, window=a.served_window)
+                                repack=not a.served_microbatches, chunk=a.served_chunk, window=a.served_window,
+                                repack_idle=a.served_idle)
     print(json.dumps(line))
 
 

@@ -212,7 +212,7 @@ int64_t bytes_of(const Sizes &z, int n)
     b += z.cells * (4 + 8 * 8 + 4 + 4 + 8 * 8 + 4 + 8 + 4 + 32); /* count, ctab, vhead, llcount, diag, cursor, near mask, listed, sums */
     b += z.cells * 8 * 8;                                 /* tapes */
     b += z.cells1 * (4 + 4);                              /* start, lloff */
-    b += z.llent * 4;
+    b += z.llent * (4 + 8);                                /* entries and their sub-cell masks */
     b += (int64_t)z.nv * (31 * 8 + 12 * 4 + IRBC_MAX_CAND * (4 + 8)); /* incl. vertex kinematics x2 */
     b += 2 * (int64_t)z.nt * (IRBC_TRI_KIN * 8 + 16) + (int64_t)z.part * 16 + z.slots * 8;
     b += (int64_t)z.ne * (5 * 4 + 2 * 8) + (int64_t)z.nt * (3 * 4 + 2 * 8);
@@ -275,6 +275,7 @@ struct irbc_ctx {
     int32_t *ccount, *cstart, *vhead, *lloff, *llcount, *llent;
     double *ctab;
     uint64_t *cmask;
+    uint64_t *llmask; /* per cell->triangle entry: the cell's sub-cells that triangle can reach (k_cell_mask) */
     int32_t *listed;  /* per replica: the cells that list a triangle, compacted */
     CellDiag diag;
     double *cell_u, *wall_z, *mem_z;
@@ -473,6 +474,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
 {
     API_GUARD;
     if (!specs || n <= 0 || !out) return fail(IRBC_ERR_ARG, "irbc_create: bad arguments");
+    if (n > IRBC_MAX_REPLICAS) return fail(IRBC_ERR_ARG, "irbc_create: at most %d replicas per context", IRBC_MAX_REPLICAS);
     *out = nullptr;
     for (int r = 0; r < n; ++r) {
         const irbc_replica_spec &s = specs[r];
@@ -541,7 +543,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     A(c->ida, z.slots) A(c->skey, z.slots) A(c->srank, z.slots) A(c->nearq, z.slots)
     A(c->deferq, 2 * z.slots)
     A(c->ccount, z.cells) A(c->csum, 4 * z.cells) A(c->cstart, z.cells1) A(c->vhead, z.cells) A(c->lloff, z.cells1)
-    A(c->llcount, z.cells) A(c->llent, z.llent) A(c->ctab, z.cells * 8) A(c->cmask, z.cells) A(c->listed, z.cells)
+    A(c->llcount, z.cells) A(c->llent, z.llent) A(c->llmask, z.llent) A(c->ctab, z.cells * 8) A(c->cmask, z.cells) A(c->listed, z.cells)
     A(c->diag.vcm, 3 * z.cells) A(c->diag.ncc, z.cells) A(c->diag.rand1, z.cells)
     A(c->diag.rand2, z.cells) A(c->diag.dvv, z.cells) A(c->diag.the, z.cells) A(c->diag.nc, z.cells)
     A(c->cell_u, 2 * z.cells) A(c->wall_z, 3 * z.cells) A(c->mem_z, 3 * z.cells)
@@ -1351,6 +1353,15 @@ static int launch_llist(irbc_ctx *c, int staged)
     return launch_listed(c, staged);
 }
 
+/* Grids of the kernels that stride over a batch's work flattened across
+ * its replicas (flat_prefix): per_rep workgroups per replica, at most
+ * per_cu per CU in all; the prefix takes nrep + 1 ints of LDS. */
+static unsigned flat_grid(const irbc_ctx *c, int64_t per_rep, int per_cu)
+{
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(per_rep * c->nrep, (int64_t)per_cu * c->ncu));
+}
+static size_t flat_lds(const irbc_ctx *c) { return (size_t)(c->nrep + 1) * sizeof(int32_t); }
+
 /* the cells' near-membrane sub-cell masks from the current lists and
  * triangle kinematics;
  * staged: every replica, else those that collide or make lists this step */
@@ -1361,11 +1372,14 @@ static int launch_cell_mask(irbc_ctx *c, int staged)
         const int rc = launch_listed(c, 1);
         if (rc) return rc;
     }
-    /* one wave per listed cell: about two and a half cells per triangle */
-    const unsigned G = (unsigned)std::max(16, c->z.maxnt * 5 / 8);
-    k_cell_mask<<<dim3(G, (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
-        c->d_rep, staged, c->par & 1, c->lloff, c->llent, c->listed, c->tri_kin + (c->par & 1) * c->tri_stride(),
-        c->tri_sph + (c->par & 1) * c->z.nt, c->cmask, c->near_masks);
+    /* one wave per listed cell, about two and a half cells per triangle,
+     * flattened over the batch: a grid of that size for one replica, at
+     * most eight workgroups per CU for a batch (they stride) */
+    const unsigned G = flat_grid(c, std::max(16, c->z.maxnt * 5 / 8), 8);
+    k_cell_mask<<<G, IRBC_WG, flat_lds(c), c->st>>>(
+        c->d_rep, c->nrep, staged, c->par & 1, c->lloff, c->llent, c->listed,
+        c->tri_kin + (c->par & 1) * c->tri_stride(), c->tri_sph + (c->par & 1) * c->z.nt, c->cmask, c->llmask,
+        c->near_masks);
     LAUNCHCHK();
     return IRBC_OK;
 }
@@ -1385,11 +1399,17 @@ static int64_t max_np(const irbc_ctx *c)
 static unsigned near_blocks(const irbc_ctx *c) { return (unsigned)std::max<int64_t>(16, (max_np(c) + 2047) / 2048); }
 
 /* the sphere pass: IRBC_SPHERE_LANES lanes per queued particle, sized for a
- * queue of up to 2% of a replica's particles in one pass */
+ * queue of up to 2% of a replica's particles in one pass, at most 16
+ * workgroups per CU for a batch (they stride over its flattened queue) */
 static unsigned sphere_blocks(const irbc_ctx *c)
 {
     const int64_t per_block = IRBC_WG / IRBC_SPHERE_LANES;
     return (unsigned)std::max<int64_t>(8, (max_np(c) / 50 + per_block - 1) / per_block);
+}
+static void launch_sphere(irbc_ctx *c, const double *tk, const float4 *ts)
+{
+    k_near_sphere<<<flat_grid(c, sphere_blocks(c), 16), IRBC_WG, flat_lds(c), c->st>>>(
+        c->d_rep, c->nrep, c->lloff, c->llent, c->llmask, c->nearq, c->deferq, tk, ts, c->par);
 }
 
 static int launch_near(irbc_ctx *c)
@@ -1402,10 +1422,9 @@ static int launch_near(irbc_ctx *c)
         LAUNCHCHK();
         return IRBC_OK;
     }
-    k_near_sphere<<<dim3(sphere_blocks(c), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
-        c->d_rep, c->lloff, c->llent, c->nearq, c->deferq, tk, ts, c->par);
-    k_near_hit<<<dim3(std::max(16u, near_blocks(c) / 8u), (unsigned)c->nrep), IRBC_WG, 0, c->st>>>(
-        c->d_rep, c->lloff, c->llent, c->deferq, tk, ts, c->par);
+    launch_sphere(c, tk, ts);
+    k_near_hit<<<flat_grid(c, std::max(16u, near_blocks(c) / 8u), 4), IRBC_WG, flat_lds(c), c->st>>>(
+        c->d_rep, c->nrep, c->lloff, c->llent, c->deferq, tk, ts, c->par);
     LAUNCHCHK();
     return IRBC_OK;
 }
@@ -2507,8 +2526,7 @@ static int launch_step_near(irbc_ctx *c, int phases)
         k_step_near<true><<<dim3(near_blocks(c) + 1, nr), IRBC_WG, 0, c->st>>>(c->d_rep, c->lloff, c->llent,
                                                                            c->nearq, tk, ts, c->par, phases, Mm);
     } else {
-        k_near_sphere<<<dim3(sphere_blocks(c), nr), IRBC_WG, 0, c->st>>>(
-            c->d_rep, c->lloff, c->llent, c->nearq, c->deferq, tk, ts, c->par);
+        launch_sphere(c, tk, ts);
         k_step_near<false><<<dim3(std::max(16u, near_blocks(c) / 8u) + 1, nr), IRBC_WG, 0, c->st>>>(
             c->d_rep, c->lloff, c->llent, c->deferq, tk, ts, c->par, phases, Mm);
     }

@@ -22,8 +22,10 @@
 #define IRBC_MEM_WG 512        /* one membrane workgroup per replica (2 waves per SIMD: no spills) */
 #define IRBC_SCAN_BLOCK 4096   /* elements per scan block (256 threads x 16) */
 #define IRBC_MAX_CAND 16       /* ligand candidates per knob (3x3 window in practice) */
+#define IRBC_TRI_KIN 18         /* doubles per triangle: 3 positions, 3 half-kicked velocities */
 #define IRBC_LLIST_PER_TRI 64  /* a triangle lies within 2a of at most 4x4x4 cells */
 #define IRBC_RANDNO 12345678   /* unbound-ligand sentinel (define.h:37) */
+#define IRBC_MAX_REPLICAS 4096 /* per context: kernels flattened over a batch keep a prefix per replica in LDS */
 #define IRBC_FIX_SCALE 1099511627776.0      /* 2^40: fixed-point unit of cell sums */
 #define IRBC_FIX_INV (1.0 / 1099511627776.0)
 #ifndef IRBC_SORT_EVERY_DEFAULT
