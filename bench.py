@@ -226,6 +226,7 @@ def batched(steps, warmup, options=-1):
     ds = en.uniform()
     b = en.batch(ds)
     kb = set_options(b, options) if options >= 0 else b.autotune(20)[0]
+    span_b = b.option(ia.OPT_BIN_SPAN)
     dt_b = timed(b, steps, warmup)
     np_b = sum(en.n_solvent(d) for d in ds)
     b.close()
@@ -242,7 +243,7 @@ def batched(steps, warmup, options=-1):
             "speedup": rb / ra, "batched_ms_per_step": dt_b * 1e3 / steps,
             "plain_ms_per_step": dt_a * 1e3 / steps,
             "batched_particle_steps_per_s": np_b * steps / dt_b,
-            "launch_options_index": {"batched": kb, "plain": ka}}
+            "launch_options_index": {"batched": kb, "plain": ka}, "bin_span": {"batched": span_b}}
 
 
 def plain_rate(ds, steps, warmup=50):
@@ -264,7 +265,7 @@ def plain_rate(ds, steps, warmup=50):
 
 
 def served(n, rate, steps, budget_gb, plain_every=8, plain_steps=1000, options=None, repack=True, chunk=250,
-           window=0.1, repack_idle=0.25):
+           window=0.1, repack_idle=0.25, bin_span=1):
     """Poisson arrivals of uneven-ensemble requests through the request
     loop, against the plain path on every plain_every-th request (a sample
     of the same mix, run one after another); contexts of several replicas
@@ -273,7 +274,7 @@ def served(n, rate, steps, budget_gb, plain_every=8, plain_steps=1000, options=N
     and finishes; else micro-batches side by side."""
     reqs = serve.poisson_requests(n, rate, steps, seed=1)
     srv = serve.Server(int(budget_gb * 2 ** 30), chunk=chunk, max_active=2, window=window, options=options,
-                       repack=repack, repack_idle=repack_idle)
+                       repack=repack, repack_idle=repack_idle, bin_span=bin_span)
     srv.run(reqs)
     out = serve.summary(reqs, srv.makespan)
     sample = [r.replica for r in reqs[::plain_every]]
@@ -285,7 +286,7 @@ def served(n, rate, steps, budget_gb, plain_every=8, plain_steps=1000, options=N
                 "repack_idle_share": srv.repack_idle if repack else None,
                 "contexts_built": srv.stats["groups"], "max_concurrent_contexts": srv.stats["max_active_groups"],
                 "max_resident_replicas": srv.stats["max_resident_replicas"],
-                "launch_options_index": options,
+                "launch_options_index": options, "bin_span": bin_span,
                 "plain_replica_steps_per_s": plain_rate(sample, plain_steps),
                 "plain_sample": "every %dth request alone (%d of them), %d timed steps each"
                                 % (plain_every, len(sample), plain_steps)})
@@ -305,6 +306,7 @@ def uneven(steps, plain_every=6, plain_steps=1000):
     b = en.batch(ds)
     b.set_option(ia.OPT_ALIGN_LISTS, 1)
     kb = b.autotune(20)[0]
+    span, span_ms = b.option(ia.OPT_BIN_SPAN), _span_ms(b)
     dt = timed(b, steps, 50)
     b.close()
     ns = sum(en.n_solvent(d) for d in ds)
@@ -316,7 +318,12 @@ def uneven(steps, plain_every=6, plain_steps=1000):
             "batched_particle_steps_per_s": ns * steps / dt, "plain_replica_steps_per_s": ra,
             "plain_sample": "every %dth replica alone (%d of them), %d timed steps each"
                             % (plain_every, len(ds[::plain_every]), plain_steps),
-            "speedup": rb / ra, "launch_options_index": kb}
+            "speedup": rb / ra, "launch_options_index": kb, "bin_span": span, "bin_span_ms_per_step": span_ms}
+
+
+def _span_ms(b):
+    """the autotuner's ms per step with bin spans 1, 2, 4 (None: not timed)"""
+    return dict((k, round(float(v), 4) if v == v else None) for k, v in zip((1, 2, 4), b.autotune_spans()))
 
 
 def trace_summary(path, skip, steps):
@@ -378,6 +385,8 @@ def main():
     ap.add_argument("--served-chunk", type=int, default=250)
     ap.add_argument("--served-window", type=float, default=0.1)
     ap.add_argument("--served-budget-gb", type=float, default=64.0)
+    ap.add_argument("--served-span", type=int, default=1, choices=(1, 2, 4),
+                    help="bin-sort tile span of contexts of several replicas (the full run takes the uneven leg's)")
     ap.add_argument("--served-idle", type=float, default=0.25,
                     help="re-pack once finished requests hold this share of the running context's particles")
     ap.add_argument("--served-microbatches", action="store_true",
@@ -408,7 +417,8 @@ def main():
         print(json.dumps({"served": served(a.served_requests, a.served_rate, a.served_steps, a.served_budget_gb,
                                            options=None if a.options < 0 else a.options,
                                            repack=not a.served_microbatches, chunk=a.served_chunk,
-                                           window=a.served_window, repack_idle=a.served_idle)}))
+                                           window=a.served_window, repack_idle=a.served_idle,
+                                           bin_span=a.served_span)}))
         return
     d = en.single()
     b = ia.Batch([en.spec(d)])
@@ -421,6 +431,7 @@ def main():
             b.set_option(opt, (tuned >> i) & 1)
     else:
         tuned, trials = b.autotune(50)  # state is restored: the trajectory does not advance
+    span, span_ms = b.option(ia.OPT_BIN_SPAN), _span_ms(b)
     b.md_steps(a.warmup)
     b.sync()
     t0 = time.perf_counter()
@@ -463,6 +474,7 @@ def main():
                               "bytes_per_particle_step": STEP_BYTES_PER_PARTICLE,
                               "ms_per_step": dt * 1e3 / a.steps}},
         "launch_options": dict([(name, (tuned >> i) & 1) for i, (name, _) in enumerate(ia.OPTIONS)],
+                               bin_span=span, bin_span_ms_per_step=span_ms,
                                autotune_ms_per_step=[round(x, 4) if x == x else None for x in trials.tolist()]),
         "phases_ms_per_launch": {
             "stream": k_ms,
@@ -482,9 +494,10 @@ def main():
         # micro-batches of several replicas take the launch shape the
         # autotuner picked for the uneven ensemble, the same mix
         opt = line["uneven"]["launch_options_index"] if "uneven" in line else None
+        a.served_span = line["uneven"]["bin_span"] if "uneven" in line else 1
         line["served"] = served(a.served_requests, a.served_rate, a.served_steps, a.served_budget_gb, options=opt,
                                 repack=not a.served_microbatches, chunk=a.served_chunk, window=a.served_window,
-                                repack_idle=a.served_idle)
+                                repack_idle=a.served_idle, bin_span=a.served_span)
     print(json.dumps(line))
 
 

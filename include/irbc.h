@@ -247,9 +247,18 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
  *   outweigh the sort's data movement), hence the default.
  *   Stage-by-stage collisions (irbc_collision_step) always sort. */
 #define IRBC_OPT_SORT_EVERY 14
+/* IRBC_OPT_BIN_SPAN  value 1 (default), 2 or 4: the bin-sort tile shape --
+ *   particle tiles (256 particles each) per binning workgroup of a sorting
+ *   collision, counted in one LDS hash so a cell costs one global atomic per
+ *   span; irbc_autotune picks it.  Collisions that run in place
+ *   (IRBC_OPT_SORT_EVERY > 1) bin one tile per workgroup.  Same results. */
+#define IRBC_OPT_BIN_SPAN 15
 int irbc_set_option(irbc_ctx *ctx, int option, int value);
+int irbc_get_option(const irbc_ctx *ctx, int option, int *value);
 /* Times `steps` MD steps under 48 combinations of the options above on
- * this context's own replicas, then keeps the fastest: the membrane as the
+ * this context's own replicas, then keeps the fastest, then on that shape
+ * times the bin spans 1, 2 and 4 (IRBC_OPT_BIN_SPAN, over at least four
+ * collisions) and keeps the fastest: the membrane as the
  * chain on one queue, the chain on its own stream or the five grid kernels
  * on its own stream, times bounce-back shape, graphs, XCD chunking and
  * stream shape.  The state is snapshot on the device first and restored
@@ -259,6 +268,9 @@ int irbc_set_option(irbc_ctx *ctx, int option, int value);
  * 128 mem_chain + 256 one_queue, NaN where not timed.  Returns the chosen
  * index or < 0.  Needs free device memory for one copy of the context. */
 int irbc_autotune(irbc_ctx *ctx, int32_t steps, double *report);
+/* the autotuner's second stage, on the launch shape it chose: ms per step
+ * with bin spans 1, 2, 4 (NaN when not timed) */
+int irbc_autotune_spans(const irbc_ctx *ctx, double *ms);
 /* keep per-cell results (irbc_get_cells) also for collisions run inside
  * irbc_md_steps; costs ~56 B per cell per collision */
 int irbc_set_diagnostics(irbc_ctx *ctx, int on);

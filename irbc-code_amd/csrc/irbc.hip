@@ -245,6 +245,8 @@ struct irbc_ctx {
     int classify_all = 0;       /* classification tests every triangle (1) or the binned ones (0) */
     int align_lists = 0;        /* list updates at collision steps (lists_due) */
     int sort_every = IRBC_SORT_EVERY_DEFAULT; /* collisions per counting sort (sorts_at) */
+    int bin_span = 1;           /* particle tiles per binning workgroup of a sorting collision (bin_span) */
+    double span_ms[3] = {NAN, NAN, NAN}; /* the autotuner's ms per step with bin spans 1, 2, 4 */
     int ncu = 256;              /* compute units of the device */
     /* the device state is as the last irbc_md_steps left it (vertex and
      * triangle kinematics, near masks, drift margins, the next step's grid
@@ -1300,7 +1302,12 @@ static int launch_collision(irbc_ctx *c, int staged, int diag, int cf)
     /* the cell-draw blocks keep the particle tiles' XCD chunking intact:
      * their count is a multiple of IRBC_NXCD */
     const unsigned ndb = xcd_grid(CT);
-    k_bin<<<nvb + ndb + (xc ? xcd_grid(T) : T), IRBC_WG, 0, c->st>>>(
+    /* the bin-sort tile shape: spans of bin_span tiles when every collision
+     * of the step sorts (the span path gives no in-place sums) */
+    const int span = (cf & COL_INPLACE) ? 1 : c->bin_span;
+    const unsigned NS = (T + (unsigned)span - 1) / (unsigned)span;
+    auto kb = span == 4 ? k_bin<4> : span == 2 ? k_bin<2> : k_bin<1>;
+    kb<<<nvb + ndb + (xc ? xcd_grid(NS) : NS), IRBC_WG, 0, c->st>>>(
         c->d_rep, c->nrep, (int)nvb, c->d_tile_rep, (int)T, xc, staged, c->skey, c->srank, c->ccount, c->csum, M,
         c->vhead, c->vnext, c->d_ctile_rep, (int)CT, (int)ndb, c->cell_u, c->ctab, c->wall_z, diag, c->diag);
     if (cf & COL_SORT) {
@@ -1894,6 +1901,39 @@ int irbc_autotune(irbc_ctx *c, int32_t steps, double *report)
         }
     }
     apply_options(c, best_k);
+    /* second stage, on the chosen launch shape: the bin-sort tile shape
+     * (tiles per binning workgroup), timed over at least four collisions */
+    if (!rc && c->sort_every == 1) {
+        int64_t n2 = steps;
+        for (const HostRep &h : c->hr) n2 = std::max<int64_t>(n2, 4 * (int64_t)h.spec.params.csteps);
+        n2 = std::min<int64_t>(n2, IRBC_TUNE_WARM_MAX);
+        double best2 = 1e300;
+        int best_span = 1;
+        for (int sp : {1, 2, 4}) {
+            c->bin_span = sp;
+            double ms = 0.0;
+            rc = snap_restore(c, S);
+            if (!rc) rc = irbc_md_steps(c, n2); /* warm-up; also captures graphs */
+            if (!rc) rc = snap_restore(c, S);
+            hipEvent_t e0 = c->event(), e1 = c->event();
+            if (!rc) rc = hipEventRecord(e0, c->st) == hipSuccess ? IRBC_OK : IRBC_ERR_HIP;
+            if (!rc) rc = irbc_md_steps(c, n2);
+            if (!rc) rc = hipEventRecord(e1, c->st) == hipSuccess ? IRBC_OK : IRBC_ERR_HIP;
+            if (!rc) rc = c->wait_all();
+            float f = 0.f;
+            if (!rc && hipEventElapsedTime(&f, e0, e1) != hipSuccess) rc = fail(IRBC_ERR_HIP, "autotune timing");
+            ms = f;
+            c->ev_pool.push_back(e0);
+            c->ev_pool.push_back(e1);
+            if (rc) break;
+            c->span_ms[sp == 1 ? 0 : sp == 2 ? 1 : 2] = ms / n2;
+            if (ms < best2) {
+                best2 = ms;
+                best_span = sp;
+            }
+        }
+        c->bin_span = best_span;
+    }
     if (!rc && c->use_graphs) {
         /* capture the chosen shape's graphs for one whole collide/list-update
          * cycle now, so that later runs replay instead of capturing */
@@ -2614,7 +2654,8 @@ static int run_graph_chunk(irbc_ctx *c, int n, uint64_t flags)
                            ((uint64_t)(c->xcd_chunk & 1) << 11) | ((uint64_t)(c->stream4 & 1) << 12) |
                            ((uint64_t)(c->stream_grid & 1) << 13) | ((uint64_t)(c->mem_after & 1) << 14) |
                            ((uint64_t)(c->mem_chain & 1) << 15) | ((uint64_t)(c->chain_dev & 1) << 16) |
-                           ((uint64_t)(c->near_masks & 1) << 17) | ((uint64_t)(c->one_queue & 1) << 18);
+                           ((uint64_t)(c->near_masks & 1) << 17) | ((uint64_t)(c->one_queue & 1) << 18) |
+                           ((uint64_t)(c->bin_span & 7) << 19);
     const std::pair<uint64_t, uint64_t> key(flags, shape);
     auto it = c->graphs.find(key);
     if (it == c->graphs.end()) {
@@ -2766,6 +2807,11 @@ int irbc_set_option(irbc_ctx *c, int option, int value)
     case IRBC_OPT_CLASSIFY_ALL:
         c->classify_all = value != 0;
         return IRBC_OK;
+    case IRBC_OPT_BIN_SPAN:
+        if (value != 1 && value != 2 && value != 4)
+            return fail(IRBC_ERR_ARG, "irbc_set_option: IRBC_OPT_BIN_SPAN is 1, 2 or 4");
+        c->bin_span = value;
+        return IRBC_OK;
     case IRBC_OPT_SORT_EVERY: {
         if (value < 1) return fail(IRBC_ERR_ARG, "irbc_set_option: IRBC_OPT_SORT_EVERY needs a value >= 1");
         HIPCHK(c->sync_both());
@@ -2798,6 +2844,37 @@ int irbc_set_option(irbc_ctx *c, int option, int value)
     default:
         return fail(IRBC_ERR_ARG, "irbc_set_option: unknown option %d", option);
     }
+}
+
+int irbc_get_option(const irbc_ctx *c, int option, int *value)
+{
+    if (!c || !value) return fail(IRBC_ERR_ARG, "irbc_get_option: bad arguments");
+    switch (option) {
+    case IRBC_OPT_MEMBRANE_FUSED: *value = c->mem_fused; break;
+    case IRBC_OPT_GRAPHS: *value = c->use_graphs; break;
+    case IRBC_OPT_NEAR_MERGED: *value = c->near_merged; break;
+    case IRBC_OPT_XCD_CHUNK: *value = c->xcd_chunk; break;
+    case IRBC_OPT_STREAM4: *value = c->stream4; break;
+    case IRBC_OPT_STREAM_GRID: *value = c->stream_grid; break;
+    case IRBC_OPT_MEM_AFTER_STREAM: *value = c->mem_after; break;
+    case IRBC_OPT_MEMBRANE_CHAIN: *value = c->mem_chain; break;
+    case IRBC_OPT_CHAIN_DEVICE_SCOPE: *value = c->chain_dev; break;
+    case IRBC_OPT_NEAR_MASKS: *value = c->near_masks; break;
+    case IRBC_OPT_ONE_QUEUE: *value = c->one_queue; break;
+    case IRBC_OPT_CLASSIFY_ALL: *value = c->classify_all; break;
+    case IRBC_OPT_ALIGN_LISTS: *value = c->align_lists; break;
+    case IRBC_OPT_SORT_EVERY: *value = c->sort_every; break;
+    case IRBC_OPT_BIN_SPAN: *value = c->bin_span; break;
+    default: return fail(IRBC_ERR_ARG, "irbc_get_option: no option %d", option);
+    }
+    return IRBC_OK;
+}
+
+int irbc_autotune_spans(const irbc_ctx *c, double *ms)
+{
+    if (!c || !ms) return fail(IRBC_ERR_ARG, "irbc_autotune_spans: bad arguments");
+    for (int k = 0; k < 3; ++k) ms[k] = c->span_ms[k];
+    return IRBC_OK;
 }
 
 int irbc_set_diagnostics(irbc_ctx *c, int on)

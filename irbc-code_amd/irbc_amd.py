@@ -29,6 +29,7 @@ EXPORTS = (
     "irbc_profile_read", "irbc_set_series", "irbc_get_series", "irbc_query", "irbc_set_option",
     "irbc_checkpoint_bytes", "irbc_checkpoint_save", "irbc_checkpoint_load", "irbc_autotune",
     "irbc_membrane_va_static", "irbc_init_solvent", "irbc_get_spec", "irbc_copy_replica",
+    "irbc_get_option", "irbc_autotune_spans",
 )
 
 OPT_MEMBRANE_FUSED = 1
@@ -45,6 +46,7 @@ OPT_ONE_QUEUE = 11
 OPT_CLASSIFY_ALL = 12
 OPT_ALIGN_LISTS = 13
 OPT_SORT_EVERY = 14
+OPT_BIN_SPAN = 15
 OPTIONS = (("membrane_fused", 1), ("near_merged", 3), ("graphs", 2), ("xcd_chunk", 4), ("stream4", 5),
            ("stream_grid", 6), ("mem_after_stream", 7), ("membrane_chain", 8), ("one_queue", 11))
 
@@ -157,6 +159,8 @@ def load():
     L.irbc_checkpoint_save.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
     L.irbc_checkpoint_load.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int64]
     L.irbc_copy_replica.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    L.irbc_get_option.argtypes = [C.c_void_p, C.c_int, _i]
+    L.irbc_autotune_spans.argtypes = [C.c_void_p, _d]
     L.irbc_membrane_va_static.argtypes = [C.c_void_p] * 5 + [C.c_int32, C.c_void_p, C.c_int32, C.c_double,
                                                              C.c_double, C.c_void_p, C.c_void_p]
     _lib = L
@@ -446,6 +450,17 @@ class Batch:
 
     def set_option(self, option, value):
         _chk(self.L.irbc_set_option(self.h, int(option), int(value)))
+
+    def option(self, option):
+        v = C.c_int32()
+        _chk(self.L.irbc_get_option(self.h, int(option), C.byref(v)))
+        return v.value
+
+    def autotune_spans(self):
+        """ms per step the autotuner measured with bin spans 1, 2, 4."""
+        ms = np.zeros(3)
+        _chk(self.L.irbc_autotune_spans(self.h, _ptr(ms, _d)))
+        return ms
 
     def set_diagnostics(self, on=True):
         _chk(self.L.irbc_set_diagnostics(self.h, 1 if on else 0))

@@ -115,15 +115,17 @@ class _Group:
             r.admitted = now
 
 
-def _configure(b, n, graphs, options):
+def _configure(b, n, graphs, options, bin_span=1):
     """Launch options of a context of n replicas: list updates beside the
     collisions (every chunk of a collision period then has one shape), and
     either a launch-option index (bit i = irbc_amd.OPTIONS[i]) for contexts
-    of several replicas or the library's defaults with graphs on or off."""
+    of several replicas, with the bin-sort tile span bin_span, or the
+    library's defaults with graphs on or off."""
     b.set_option(ia.OPT_ALIGN_LISTS, 1)
     if options is not None and n >= 2:
         for i, (_, opt) in enumerate(ia.OPTIONS):
             b.set_option(opt, (options >> i) & 1)
+        b.set_option(ia.OPT_BIN_SPAN, bin_span)
     else:
         b.set_option(ia.OPT_GRAPHS, 1 if graphs else 0)
         if n >= 8:
@@ -157,7 +159,7 @@ class _Slots:
     switch: the still-running requests' from the context before, the new
     ones' from their staging contexts."""
 
-    def __init__(self, cont, new, graphs, options):
+    def __init__(self, cont, new, graphs, options, bin_span=1):
         self.reqs = list(cont) + list(new)
         self.ncont = len(cont)
         self.left = [0] * len(self.reqs)
@@ -165,7 +167,7 @@ class _Slots:
         t = time.perf_counter()
         b = ia.Batch([r.spec for r in self.reqs])
         try:
-            _configure(b, len(self.reqs), graphs, options)
+            _configure(b, len(self.reqs), graphs, options, bin_span)
         except BaseException:
             b.close()
             raise
@@ -180,7 +182,7 @@ class _Slots:
 
 class Server:
     def __init__(self, budget_bytes, chunk=250, max_group=64, max_active=2, window=0.0, builders=1, graphs=False,
-                 options=None, repack=True, repack_idle=0.25, stagers=4):
+                 options=None, repack=True, repack_idle=0.25, stagers=4, bin_span=1):
         """budget_bytes: device bytes the loop may hold (re-packing holds the
         running context, the next one and the staged requests at a switch);
         chunk: steps per irbc_md_steps call; max_group: replicas per
@@ -188,7 +190,10 @@ class Server:
         options: launch-option index for contexts of several replicas;
         repack_idle: re-pack once finished requests hold this share of the
         running context's particles; stagers: threads setting up arrived
-        requests, each in a context of its own, while the GPU runs."""
+        requests, each in a context of its own, while the GPU runs;
+        bin_span: the bin-sort tile span of contexts of several replicas
+        that take `options`."""
+        self.bin_span = int(bin_span)
         self.repack = bool(repack)
         self.repack_idle = float(repack_idle)
         self.stagers = int(stagers)
@@ -371,7 +376,7 @@ class Server:
         self.used += need
         self.stats["groups"] += 1
         self.log.append({"planned": round(now, 4), "cont": len(cont), "new": len(new)})
-        return (self.pool.submit(_Slots, cont, new, self.graphs, self.options), cont, new, need)
+        return (self.pool.submit(_Slots, cont, new, self.graphs, self.options, self.bin_span), cont, new, need)
 
     def _read(self, ctx, i):
         r = ctx.reqs[i]

@@ -207,7 +207,8 @@ def test_sort_schedule_never_changes_results():
     ds = _mix() + [en.replica(20, 20, 16, 10, 642, None, 1.0, 300, kon=5.0, koff=1.0, shear=0.25, seed=31)]
     runs = []
     for every, opts in ((1, {}), (3, {}), (3, {ia.OPT_STREAM4: 1}),
-                        (3, {ia.OPT_ONE_QUEUE: 0, ia.OPT_GRAPHS: 0}), (1000, {ia.OPT_STREAM4: 1})):
+                        (3, {ia.OPT_ONE_QUEUE: 0, ia.OPT_GRAPHS: 0}), (1000, {ia.OPT_STREAM4: 1}),
+                        (3, {ia.OPT_BIN_SPAN: 4})):
         print("sort every", every, opts, flush=True)
         b = en.batch(ds)
         b.set_option(ia.OPT_SORT_EVERY, every)
@@ -392,3 +393,31 @@ def test_lists_at_collisions_never_change_results():
             if key == "counters":
                 continue  # the list counters differ by design
             np.testing.assert_array_equal(snaps[0][r][key], snaps[1][r][key], err_msg="replica %d %s" % (r, key))
+
+
+@pytest.mark.gpu
+def test_bin_span_does_not_change_results():
+    """The bin-sort tile shape (1, 2 or 4 particle tiles per binning
+    workgroup, spans crossing from one replica into the next in a batch of
+    uneven sizes) gives the same bits over 41 steps (four collisions, list
+    updates), with and without XCD chunking and graphs."""
+    ds = _mix() + [en.replica(20, 20, 16, 10, 642, None, 1.0, 300, kon=5.0, koff=1.0, shear=0.25, seed=31)]
+    snaps = []
+    for span, xcd, graphs in ((1, 1, 1), (2, 1, 1), (4, 1, 1), (4, 0, 0), (2, 0, 1)):
+        b = en.batch(ds)
+        b.set_option(ia.OPT_BIN_SPAN, span)
+        b.set_option(ia.OPT_XCD_CHUNK, xcd)
+        b.set_option(ia.OPT_GRAPHS, graphs)
+        assert b.option(ia.OPT_BIN_SPAN) == span
+        b.md_steps(41)
+        b.sync()
+        snaps.append([_snapshot(b, r) for r in range(len(ds))])
+        b.close()
+    for other in snaps[1:]:
+        for r in range(len(ds)):
+            for key in snaps[0][r]:
+                np.testing.assert_array_equal(snaps[0][r][key], other[r][key], err_msg="replica %d %s" % (r, key))
+    b = en.batch(ds[:1])
+    with pytest.raises(ia.IrbcError):
+        b.set_option(ia.OPT_BIN_SPAN, 3)
+    b.close()
