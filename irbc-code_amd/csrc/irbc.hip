@@ -1531,7 +1531,7 @@ static int membrane_launch(irbc_ctx *c, int phases, hipStream_t s, int k)
         LAUNCHCHK();
         return IRBC_OK;
     }
-    const unsigned X = IRBC_NXCD;
+    const unsigned X = IRBC_MEM_XCD_AFFINE ? IRBC_NXCD : 1;
     const dim3 gv(X * (unsigned)((c->z.maxnv + IRBC_WG - 1) / IRBC_WG), nr);
     /* element blocks, then vertex blocks for the bond preparation */
     const dim3 ge(X * (unsigned)((std::max(c->z.maxne, c->z.maxnt) + IRBC_WG - 1) / IRBC_WG +
