@@ -298,19 +298,23 @@ def served(n, rate, steps, budget_gb, plain_every=8, plain_steps=1000, options=N
     return out
 
 
-def uneven(steps, plain_every=6, plain_steps=1000):
+def uneven(steps, plain_every=6, plain_steps=1000, options=-1):
     """48 replicas of mixed boxes, meshes, rates and shears in one batch
-    (list updates at collision steps), against the plain path on every
-    plain_every-th of them."""
+    (list updates at collision steps), autotuned (or with the launch-option
+    index `options`), against the plain path on every plain_every-th of
+    them (plain_every 0: no plain sample, for profiling the batch alone)."""
     ds = en.uneven()
     b = en.batch(ds)
     b.set_option(ia.OPT_ALIGN_LISTS, 1)
-    kb = b.autotune(20)[0]
+    kb = set_options(b, options) if options >= 0 else b.autotune(20)[0]
     span, span_ms = b.option(ia.OPT_BIN_SPAN), _span_ms(b)
     dt = timed(b, steps, 50)
     b.close()
     ns = sum(en.n_solvent(d) for d in ds)
     rb = len(ds) * steps / dt
+    if plain_every <= 0:
+        return {"steps": steps, "batched_ms_per_step": dt * 1e3 / steps, "batched_replica_steps_per_s": rb,
+                "batched_particle_steps_per_s": ns * steps / dt, "launch_options_index": kb, "bin_span": span}
     ra = plain_rate(ds[::plain_every], plain_steps)
     return {"workload": "48 replicas, boxes 24^3..64x64x48 at density 5-10 (%d particles), 162/642/2562-vertex "
                         "ellipsoids, 100-2000 knobs, shear 0.01-0.1" % ns,
@@ -375,6 +379,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=2000)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--uneven-steps", type=int, default=1000)
+    ap.add_argument("--uneven-plain-every", type=int, default=6,
+                    help="time every n-th replica of the uneven batch alone (0: none)")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the 64-replica batch against its first replica alone (for profiling)")
     ap.add_argument("--only", choices=("batched", "uneven", "served"),
@@ -411,7 +417,7 @@ def main():
         print(json.dumps({"batched": batched(a.batched_steps, 100, a.options)}))
         return
     if a.only == "uneven":
-        print(json.dumps({"uneven": uneven(a.uneven_steps)}))
+        print(json.dumps({"uneven": uneven(a.uneven_steps, plain_every=a.uneven_plain_every, options=a.options)}))
         return
     if a.only == "served":
         print(json.dumps({"served": served(a.served_requests, a.served_rate, a.served_steps, a.served_budget_gb,

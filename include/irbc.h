@@ -8,8 +8,12 @@
  * launches hand-written HIP kernels for gfx950.
  *
  * Every call returns an int status: IRBC_OK (0) or a negative IRBC_ERR_*.
- * No exceptions, no exit().  A context is used by one host thread; calls are
- * ordered on the context's HIP stream.
+ * No exceptions, no exit().  A context is used by one host thread at a time;
+ * calls are ordered on the context's HIP streams.  Different contexts may be
+ * driven from different threads at once: a graph capture inside
+ * irbc_md_steps excludes the other threads' calls for its duration.  Context
+ * memory is stream-ordered from the device's default pool, which keeps freed
+ * memory for the next context.
  *
  * Reference surface replaced (citations are to the reference sources):
  *   define.h:35-73 macros + maincode.cpp literals   -> irbc_params (irbc_params.h)
