@@ -208,7 +208,7 @@ Sizes plan(const irbc_replica_spec *specs, int n, std::vector<DevRep> *reps,
 int64_t bytes_of(const Sizes &z, int n)
 {
     int64_t b = 0;
-    b += z.slots * (6 * 4 + 4 + 32 + 3 * 4);              /* particles, ids, sort records, keys, ranks, queue */
+    b += z.slots * (6 * 4 + 4 + 32 + 3 * 4 + 32);         /* particles, ids, sort records, keys, ranks, queue, its records */
     b += z.cells * (4 + 8 * 8 + 4 + 4 + 8 * 8 + 4 + 8 + 4 + 32); /* count, ctab, vhead, llcount, diag, cursor, near mask, listed, sums */
     b += z.cells * 8 * 8;                                 /* tapes */
     b += z.cells1 * (4 + 4);                              /* start, lloff */
@@ -272,6 +272,7 @@ struct irbc_ctx {
     uint32_t *ida;
     float *rec; /* the collision's sorted records, 8 floats per slot */
     int32_t *skey, *srank, *nearq, *deferq;
+    float4 *nrec; /* queue records beside nearq, 2 per slot (DevRep::nrec) */
     unsigned long long *csum; /* per cell: fixed-point momentum x y z and dvv of an in-place collision */
     /* cells */
     int32_t *ccount, *cstart, *vhead, *lloff, *llcount, *llent;
@@ -543,7 +544,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
     A(c->xa, z.slots) A(c->ya, z.slots) A(c->za, z.slots) A(c->vxa, z.slots) A(c->vya, z.slots) A(c->vza, z.slots)
     A(c->rec, 8 * z.slots)
     A(c->ida, z.slots) A(c->skey, z.slots) A(c->srank, z.slots) A(c->nearq, z.slots)
-    A(c->deferq, 2 * z.slots)
+    A(c->deferq, 2 * z.slots) A(c->nrec, 2 * z.slots)
     A(c->ccount, z.cells) A(c->csum, 4 * z.cells) A(c->cstart, z.cells1) A(c->vhead, z.cells) A(c->lloff, z.cells1)
     A(c->llcount, z.cells) A(c->llent, z.llent) A(c->llmask, z.llent) A(c->ctab, z.cells * 8) A(c->cmask, z.cells) A(c->listed, z.cells)
     A(c->diag.vcm, 3 * z.cells) A(c->diag.ncc, z.cells) A(c->diag.rand1, z.cells)
@@ -602,6 +603,7 @@ int irbc_create(const irbc_replica_spec *specs, int n, int device, irbc_ctx **ou
         R.rec = c->rec;
         R.rec1 = c->rec + 4 * c->z.slots;
         R.skeya = c->skey;
+        R.nrec = (float *)c->nrec;
         R.sort_every = c->sort_every;
         R.key_src = 1;
         top_of_step(p, 0, -1, 0, R);

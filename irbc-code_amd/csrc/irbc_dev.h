@@ -159,6 +159,10 @@ struct DevRep {
     /* A collision that does not sort leaves the particles in their slots
      * and writes each one's shifted-cell key here, slot by slot. */
     int32_t *skeya;
+    /* Beside each bounce-back queue entry, by queue position, the particle as
+     * the streaming pass saw it: two 16-byte words (x y z vx) (vy vz, slot,
+     * cell | inside flag << 31), so the sphere pass reads them coalesced */
+    float *nrec;
     int32_t skey_pad[2];
 };
 
