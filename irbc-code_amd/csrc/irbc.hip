@@ -1374,6 +1374,9 @@ static size_t flat_lds(const irbc_ctx *c) { return (size_t)(c->nrep + 1) * sizeo
 /* the cells' near-membrane sub-cell masks from the current lists and
  * triangle kinematics;
  * staged: every replica, else those that collide or make lists this step */
+#ifndef IRBC_MASK_PER_CU
+#define IRBC_MASK_PER_CU 8
+#endif
 static int launch_cell_mask(irbc_ctx *c, int staged)
 {
     if (staged) {
@@ -1384,7 +1387,7 @@ static int launch_cell_mask(irbc_ctx *c, int staged)
     /* one wave per listed cell, about two and a half cells per triangle,
      * flattened over the batch: a grid of that size for one replica, at
      * most eight workgroups per CU for a batch (they stride) */
-    const unsigned G = flat_grid(c, std::max(16, c->z.maxnt * 5 / 8), 8);
+    const unsigned G = flat_grid(c, std::max(16, c->z.maxnt * 5 / 8), IRBC_MASK_PER_CU);
     k_cell_mask<<<G, IRBC_WG, flat_lds(c), c->st>>>(
         c->d_rep, c->nrep, staged, c->par & 1, c->lloff, c->llent, c->listed,
         c->tri_kin + (c->par & 1) * c->tri_stride(), c->tri_sph + (c->par & 1) * c->z.nt, c->cmask, c->llmask,
@@ -1410,6 +1413,12 @@ static unsigned near_blocks(const irbc_ctx *c) { return (unsigned)std::max<int64
 /* the sphere pass: IRBC_SPHERE_LANES lanes per queued particle, sized for a
  * queue of up to 2% of a replica's particles in one pass, at most 16
  * workgroups per CU for a batch (they stride over its flattened queue) */
+#ifndef IRBC_SPHERE_PER_CU
+#define IRBC_SPHERE_PER_CU 16
+#endif
+#ifndef IRBC_HIT_PER_CU
+#define IRBC_HIT_PER_CU 4
+#endif
 static unsigned sphere_blocks(const irbc_ctx *c)
 {
     const int64_t per_block = IRBC_WG / IRBC_SPHERE_LANES;
@@ -1417,7 +1426,7 @@ static unsigned sphere_blocks(const irbc_ctx *c)
 }
 static void launch_sphere(irbc_ctx *c, const double *tk, const float4 *ts)
 {
-    k_near_sphere<<<flat_grid(c, sphere_blocks(c), 16), IRBC_WG, flat_lds(c), c->st>>>(
+    k_near_sphere<<<flat_grid(c, sphere_blocks(c), IRBC_SPHERE_PER_CU), IRBC_WG, flat_lds(c), c->st>>>(
         c->d_rep, c->nrep, c->lloff, c->llent, c->llmask, c->nearq, c->deferq, tk, ts, c->par);
 }
 
@@ -1432,7 +1441,7 @@ static int launch_near(irbc_ctx *c)
         return IRBC_OK;
     }
     launch_sphere(c, tk, ts);
-    k_near_hit<<<flat_grid(c, std::max(16u, near_blocks(c) / 8u), 4), IRBC_WG, flat_lds(c), c->st>>>(
+    k_near_hit<<<flat_grid(c, std::max(16u, near_blocks(c) / 8u), IRBC_HIT_PER_CU), IRBC_WG, flat_lds(c), c->st>>>(
         c->d_rep, c->nrep, c->lloff, c->llent, c->deferq, tk, ts, c->par);
     LAUNCHCHK();
     return IRBC_OK;
