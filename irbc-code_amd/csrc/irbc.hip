@@ -1523,7 +1523,8 @@ static unsigned chain_blocks(const irbc_ctx *c)
     const unsigned tb = (unsigned)((c->z.maxnt + IRBC_WG - 1) / IRBC_WG);
     const unsigned VPB = IRBC_WG / IRBC_GATHER_LANES; /* gather: vertices per workgroup */
     const unsigned gb = (unsigned)((c->z.maxnv + VPB - 1) / VPB);
-    return vb + (eb + tb + vb) + gb + 1;
+    /* begin (none when the gather folds it in), elements, gather, finish */
+    return (IRBC_CHAIN_FOLD_BEGIN ? 0u : vb) + (eb + tb + vb) + gb + 1;
 }
 
 static int membrane_launch(irbc_ctx *c, int phases, hipStream_t s, int k)
