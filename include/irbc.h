@@ -260,9 +260,10 @@ int irbc_md_steps(irbc_ctx *ctx, int64_t nsteps);
 int irbc_set_option(irbc_ctx *ctx, int option, int value);
 int irbc_get_option(const irbc_ctx *ctx, int option, int *value);
 /* Times `steps` MD steps under 48 combinations of the options above on
- * this context's own replicas, then keeps the fastest, then on that shape
- * times the bin spans 1, 2 and 4 (IRBC_OPT_BIN_SPAN, over at least four
- * collisions) and keeps the fastest: the membrane as the
+ * this context's own replicas, times the four fastest again twice each
+ * over at least four collisions and keeps the fastest of those, then on
+ * that shape times the bin spans 1, 2 and 4 (IRBC_OPT_BIN_SPAN, over at
+ * least four collisions) and keeps the fastest: the membrane as the
  * chain on one queue, the chain on its own stream or the five grid kernels
  * on its own stream, times bounce-back shape, graphs, XCD chunking and
  * stream shape.  The state is snapshot on the device first and restored
