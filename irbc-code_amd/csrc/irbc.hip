@@ -1318,7 +1318,8 @@ static int launch_collision(irbc_ctx *c, int staged, int diag, int cf)
         k_scatter<<<xc ? xcd_grid(T) : T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, xc, staged, c->skey,
                                                                c->srank, c->cstart, -1);
     }
-    k_cell<<<xc ? xcd_grid(CT) : CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, xc, staged, c->cstart,
+    auto kc = (cf & COL_INPLACE) && !staged ? k_cell<true> : k_cell<false>;
+    kc<<<xc ? xcd_grid(CT) : CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, xc, staged, c->cstart,
                                                          c->vhead, c->vnext, c->mvx, c->mvy, c->mvz, c->wall_z,
                                                          c->mem_z, c->ctab, diag, c->diag, c->par, c->ccount,
                                                          c->csum);
