@@ -1318,11 +1318,13 @@ static int launch_collision(irbc_ctx *c, int staged, int diag, int cf)
         k_scatter<<<xc ? xcd_grid(T) : T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, xc, staged, c->skey,
                                                                c->srank, c->cstart, -1);
     }
-    auto kc = (cf & COL_INPLACE) && !staged ? k_cell<true> : k_cell<false>;
+    /* without in-place replicas k_cell also rotates the vertices and parasites (k_vrot's work) */
+    const bool inpl = (cf & COL_INPLACE) && !staged;
+    auto kc = inpl ? k_cell<true> : k_cell<false>;
     kc<<<xc ? xcd_grid(CT) : CT, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_ctile_rep, (int)CT, xc, staged, c->cstart,
                                                          c->vhead, c->vnext, c->mvx, c->mvy, c->mvz, c->wall_z,
                                                          c->mem_z, c->ctab, diag, c->diag, c->par, c->ccount,
-                                                         c->csum);
+                                                         c->csum, M, inpl ? 0 : 1);
     if (cf & COL_INPLACE) {
         k_dvv<<<xc ? xcd_grid(T) : T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, (int)T, xc, c->skey, c->ctab,
                                                            c->csum);
@@ -1330,7 +1332,7 @@ static int launch_collision(irbc_ctx *c, int staged, int diag, int cf)
                                                                  c->vnext, c->mvx, c->mvy, c->mvz, c->ctab, diag,
                                                                  c->diag, c->ccount, c->csum);
     }
-    k_vrot<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, staged, M, c->ctab);
+    if (inpl) k_vrot<<<c->nrep, IRBC_WG, 0, c->st>>>(c->d_rep, staged, M, c->ctab);
     if (staged) k_rotate<<<T, IRBC_WG, 0, c->st>>>(c->d_rep, c->d_tile_rep, c->ctab);
     LAUNCHCHK();
     return IRBC_OK;

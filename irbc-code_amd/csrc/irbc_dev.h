@@ -273,6 +273,27 @@ __device__ __forceinline__ int32_t cell_of(int fx, int fy, int fz, int Lx, int L
  * cache lines (neighbouring cells of a sort) come from one XCD's L2.
  * Blocks with xcd_tile(n) >= n do nothing.  With chunked = 0 the block
  * index is the tile (a grid of exactly n blocks). */
+/* the membrane arrays as the kernels see them, for one parity of the
+ * double-buffered kinematics */
+struct MemDev {
+    double *x, *y, *z, *vx, *vy, *vz, *fx, *fy, *fz, *poff, *pon;
+    int32_t *ix, *iy, *pid, *bondi, *bondl, *knob;
+    const int32_t *e1, *e2, *es1, *es2, *iedge, *t1, *t2, *t3;
+    const double *l0, *kappap, *tarea;
+    double *tri_area, *wca, *slot, *upos, *part, *tri_kin;
+    double *tag;       /* per slot: NaN (a finished term), a triangle's area (the global-area term's vector
+                        * C, to be scaled by beta / (4 area)) or -1 (the volume term's vector w, to be
+                        * scaled as betav w / 6) */
+    int64_t upos_base; /* vertex g's unwrapped position at upos[4 (g - upos_base)] */
+    double *vkin;      /* per vertex: position, velocity after the coming first half-kick */
+    float4 *tri_sph;  /* per triangle: bounding-sphere centre and reach for the coming step */
+    const int32_t *inc_off, *inc_ent;
+    int32_t *ncand, *cand, *draw_off, *flag, *list;
+    int32_t *ligi;
+    const double *ligx, *ligy;
+    const double *bond_u;
+};
+
 #define IRBC_NXCD 8
 __device__ __forceinline__ int xcd_tile_of(int n, int chunked, int b)
 {
